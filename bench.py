@@ -1,0 +1,189 @@
+"""Benchmark: exact fault-stiffness K build at dz = 50 m, L_fault = 800, L_normal = 400.
+
+One step = one full :exact K build (2178 unit-slip columns, l_f = L_fault = 800 m,
+58,806 DOF) by the mixed-precision batched CG (FP32 stencil/CG inside FP64 reliable
+updates, rtol 1e-10), with the operator resident on the device and K left in device
+memory. Prints one JSON line (see the keys below).
+
+Besides the timed steps it measures, on the same GPU: the plain one-RHS FP64 CG
+(host alpha/beta, no graphs) on a column sample, the batched FP64 build, K's error
+against the batched FP64 K and against the CPU oracle on sampled columns, the
+stencil kernel's average launch duration (HIP events on the solver's stream) for the
+roofline, and the CPU oracle's threaded build on a column sample.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "earthquakediffinitive.jl_amd")]
+
+import eqd  # noqa: E402
+from eqd import _lib  # noqa: E402
+
+HBM_PEAK = 8.0e12
+
+
+class DeviceBuffer:
+    """A raw device allocation (hipMalloc) for K kept on the GPU."""
+
+    def __init__(self, nbytes):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.ptr = ctypes.c_void_p()
+        rc = self.hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(nbytes))
+        if rc != 0:
+            raise RuntimeError(f"hipMalloc failed: {rc}")
+        self.nbytes = nbytes
+
+    def to_host(self, shape):
+        out = np.zeros(shape, order="F")
+        rc = self.hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), self.ptr, ctypes.c_size_t(self.nbytes), 2)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed: {rc}")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = None
+
+
+def build_device(solver, Kbuf, nf2):
+    _lib.check(_lib.lib().eqd_fault_stiffness_device(solver.handle, _lib.STIFFNESS_EXACT, None, 0, Kbuf.ptr, nf2))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--dz", type=float, default=50.0)
+    ap.add_argument("--L_fault", type=float, default=800.0)
+    ap.add_argument("--L_normal", type=float, default=400.0)
+    ap.add_argument("--l_f", type=float, default=800.0)
+    ap.add_argument("--tile_k", type=int, default=64)
+    ap.add_argument("--plain_cols", type=int, default=16)
+    ap.add_argument("--cpu_cols", type=int, default=64)
+    ap.add_argument("--no_cpu", action="store_true")
+    ap.add_argument("--no_extras", action="store_true", help="only the timed mixed-precision steps")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the K build shards by columns "
+              f"across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    mu, nu = 32.04e9, 0.25
+    lam = 2 * mu * nu / (1 - 2 * nu)
+    g = eqd.bp8_grid_pair(args.dz, args.L_fault, args.L_normal)
+    fe = eqd.FaultElasticity(g, lam, mu, l_f=args.l_f, precision="mixed", tile_k=args.tile_k)
+    nf = eqd.frictional_node_count(fe)
+    nf2 = 2 * nf
+    ncols = nf2
+    dof = fe.Ntot
+    nred = fe.op.nreduced
+    print(f"bench: dz={args.dz} L=({args.L_fault},{args.L_normal}) l_f={args.l_f}: DOF {dof}, "
+          f"device unknowns {nred}, nf {nf}, {ncols} columns", file=sys.stderr, flush=True)
+    Kbuf = DeviceBuffer(nf2 * nf2 * 8)
+
+    for i in range(args.warmup):
+        build_device(fe.rs, Kbuf, nf2)
+        print(f"bench: warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    times = []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        build_device(fe.rs, Kbuf, nf2)
+        times.append(time.perf_counter() - t0)
+        print(f"bench: step {i + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    ms_per_step = 1e3 * sum(times) / len(times)
+    value = ncols / (ms_per_step / 1e3)
+    K_mixed = Kbuf.to_host((nf2, nf2))
+    rep = eqd.solver_report(fe.rs)
+    iters_per_col = rep["iterations"] / rep["solves"]
+
+    out = {
+        "metric": "K-columns/s at dz=50 to 1e-8 rel (mixed block CG vs FP64); % HBM peak; p99 lat",
+        "value": round(value, 2), "unit": "K-columns/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 CG in fp64 reliable updates",
+        "data": "synthetic: unit-slip right-hand sides of the exact K build",
+        "config": {"workload": f"exact K build, dz={args.dz:g} m, L_fault={args.L_fault:g}, "
+                               f"L_normal={args.L_normal:g}, l_f={args.l_f:g}",
+                   "columns": ncols, "dof": dof, "device_unknowns": nred, "tile_k": args.tile_k,
+                   "rtol": 1e-10, "mean_cg_iterations": round(iters_per_col, 2)},
+    }
+
+    if not args.no_extras:
+        # batched FP64 (device alpha/beta) on the same GPU
+        fe64 = eqd.FaultElasticity(g, lam, mu, l_f=args.l_f, precision="fp64", tile_k=args.tile_k)
+        t0 = time.perf_counter()
+        build_device(fe64.rs, Kbuf, nf2)
+        t64 = time.perf_counter() - t0
+        K64 = Kbuf.to_host((nf2, nf2))
+        err_vs_fp64 = float(np.linalg.norm(K_mixed - K64) / np.linalg.norm(K64))
+        print(f"bench: batched fp64 build {t64 * 1e3:.1f} ms, mixed vs fp64 K rel err {err_vs_fp64:.2e}",
+              file=sys.stderr, flush=True)
+        # plain one-RHS FP64 CG, host alpha/beta, on a column sample
+        rng = np.random.default_rng(20261203)
+        sample = np.sort(rng.choice(ncols, size=min(args.plain_cols, ncols), replace=False))
+        fep = eqd.FaultElasticity(g, lam, mu, l_f=args.l_f, engine="plain")
+        t0 = time.perf_counter()
+        Kp = eqd.fault_stiffness(fep, cols=sample)
+        tp = time.perf_counter() - t0
+        plain_value = len(sample) / tp
+        err_plain = float(np.linalg.norm(Kp - K64[:, sample]) / np.linalg.norm(K64[:, sample]))
+        print(f"bench: plain fp64 {len(sample)} cols in {tp:.2f} s ({plain_value:.1f} cols/s), "
+              f"vs batched fp64 {err_plain:.2e}", file=sys.stderr, flush=True)
+        # stencil kernel timing (events around every launch) for the roofline
+        fet = eqd.FaultElasticity(g, lam, mu, l_f=args.l_f, precision="mixed", tile_k=args.tile_k, timing=1)
+        build_device(fet.rs, Kbuf, nf2)
+        names = ["stencil", "update", "pupdate", "finalize", "xaccum", "residual64"]
+        kt = {n: fet.rs.kernel_timing(i) for i, n in enumerate(names)}
+        st_ms, st_n, st_tiles = kt["stencil"]
+        avg_us = 1e3 * st_ms / st_n
+        # algorithmic bytes of q = A p per launch: read p, write q (FP32) for every active column
+        bytes_per_launch = (st_tiles / st_n) * args.tile_k * nred * 4 * 2
+        achieved = bytes_per_launch / (avg_us * 1e-6)
+        tot_ms = sum(v[0] for v in kt.values())
+        # whole-iteration metric of the survey: 10 vector passes x DOF x 4 B per column-iteration
+        col_iters = rep["iterations"] / max(1, rep["solves"]) * ncols
+        survey_gbs = 10 * dof * 4 * col_iters / (ms_per_step / 1e3)
+        out["roofline"] = {"bound": "hbm", "kernel": f"k_apply_dot<float,{args.tile_k}>",
+                           "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                           "avg_launch_us": round(avg_us, 2), "launches": st_n,
+                           "bytes_per_launch": int(bytes_per_launch)}
+        out["kernel_share"] = {n: round(v[0] / tot_ms, 4) for n, v in kt.items()}
+        out["survey_algorithmic_GBps"] = round(survey_gbs / 1e9, 1)
+        out["plain_fp64_one_rhs"] = {"value": round(plain_value, 2), "unit": "K-columns/s",
+                                     "sample_columns": len(sample)}
+        out["batched_fp64"] = {"value": round(ncols / t64, 2), "unit": "K-columns/s"}
+        out["speedup_vs_plain_fp64"] = round(value / plain_value, 1)
+        out["K_rel_err_vs_fp64"] = err_vs_fp64
+
+        if not args.no_cpu:
+            from oracle.oracle import OracleSystem, bp8_geometry
+            o = OracleSystem(bp8_geometry(args.dz, args.L_fault, args.L_normal, args.l_f))
+            ncpu = min(os.cpu_count() or 1, 16)
+            csample = np.sort(rng.choice(ncols, size=min(args.cpu_cols, ncols), replace=False))
+            t0 = time.perf_counter()
+            Kc = o.fault_stiffness_cols(csample, nthreads=ncpu)
+            tc = time.perf_counter() - t0
+            err_oracle = float(np.linalg.norm(K_mixed[:, csample] - Kc) / np.linalg.norm(Kc))
+            out["cpu_baseline"] = {"value": round(len(csample) / tc, 3), "unit": "K-columns/s", "cores": ncpu,
+                                   "kind": "port",
+                                   "sample": f"{len(csample)} seeded random columns of the same K build, "
+                                             f"assembled-CSR FP64 CG, one column per thread"}
+            out["K_rel_err_vs_oracle_sample"] = err_oracle
+            print(f"bench: cpu oracle {len(csample)} cols in {tc:.1f} s on {ncpu} threads; "
+                  f"mixed K vs oracle {err_oracle:.2e}", file=sys.stderr, flush=True)
+    Kbuf.free()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

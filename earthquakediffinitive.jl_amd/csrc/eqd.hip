@@ -1,0 +1,1101 @@
+// libeqd host orchestration and C ABI (see include/eqd.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/eqd.h"
+#include "eqd_internal.h"
+#include "kernels.h"
+
+using namespace eqd;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+    eqd_status code;
+    Error(eqd_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                                 \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            throw Error(e_ == hipErrorOutOfMemory ? EQD_ERR_NOMEM : EQD_ERR_DEVICE,                  \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                          \
+    } while (0)
+
+template <typename F>
+eqd_status guard(F&& f) {
+    try {
+        f();
+        return EQD_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return EQD_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return EQD_ERR_ARG;
+    }
+}
+
+// Owning device allocation.
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    explicit DevMem(size_t n) { alloc(n); }
+    void alloc(size_t n) {
+        release();
+        if (n) HIPCHK(hipMalloc(&p, n));
+        bytes = n;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~DevMem() { release(); }
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+void check_device() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw Error(EQD_ERR_DEVICE, "no HIP device is visible");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        throw Error(EQD_ERR_DEVICE, std::string("libeqd is built for gfx950, device is ") + prop.gcnArchName);
+}
+
+void validate_grid(const eqd_grid_desc& g) {
+    if (g.n1 < 9 || g.n23 < 9)
+        throw Error(EQD_ERR_GRID, "SBP order 4 needs at least 9 points per dimension (n1 = " + std::to_string(g.n1) +
+                                      ", n23 = " + std::to_string(g.n23) + ")");
+    if (!(g.L_normal > 0) || !(g.L_fault > 0) || !(g.mu > 0)) throw Error(EQD_ERR_ARG, "L_normal, L_fault, mu must be > 0");
+}
+
+// Omega_f from l_f as src/FaultResponse.jl:114-126 finds it.
+void omega_range(const eqd_grid_desc& g, int& lo, int& cnt) {
+    const double h2 = 2.0 * g.L_fault / (g.n23 - 1);
+    const double tol = 1e-9 * std::max(g.l_f, 1.0);
+    lo = -1;
+    cnt = 0;
+    for (int i = 0; i < g.n23; ++i) {
+        double x = -g.L_fault + i * h2;
+        if (std::fabs(x) <= g.l_f + tol) {
+            if (lo < 0) lo = i;
+            ++cnt;
+        }
+    }
+    if (cnt == 0) throw Error(EQD_ERR_GRID, "no fault nodes inside Omega_f");
+    double xlo = -g.L_fault + lo * h2, xhi = -g.L_fault + (lo + cnt - 1) * h2;
+    if (!(std::fabs(xlo + g.l_f) <= tol && std::fabs(xhi - g.l_f) <= tol))
+        throw Error(EQD_ERR_GRID, "the elastic grid has no nodes exactly on x = +-l_f; pick a spacing that divides l_f");
+}
+
+}  // namespace
+
+struct eqd_op {
+    eqd_grid_desc g{};
+    HostTables tb;
+    int NN = 0, nf = 0, n2f = 0, i2lo = 0;
+    int64_t ntot = 0, N = 0;
+    std::vector<std::unique_ptr<DevMem>> mem;
+    OpView<double> v64{};
+    OpView<float> v32{};
+    RhsView rv{};
+    ExtractView ev{};
+    std::atomic<int> refs{1};
+
+    template <typename V>
+    V* up(const std::vector<V>& h) {
+        mem.emplace_back(new DevMem(h.size() * sizeof(V)));
+        HIPCHK(hipMemcpy(mem.back()->p, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice));
+        return mem.back()->as<V>();
+    }
+    template <typename V>
+    Tab<V> tab(const HostTab& h) {
+        std::vector<V> c(h.c.begin(), h.c.end());
+        return Tab<V>{up(h.start), up(h.width), up(c)};
+    }
+    template <typename V>
+    OpView<V> view() {
+        std::vector<V> hx(tb.Hx.begin(), tb.Hx.end()), hy(tb.Hy.begin(), tb.Hy.end());
+        return OpView<V>{tb.NX, tb.NY, tb.mface, tab<V>(tb.AXs), tab<V>(tb.AXo), tab<V>(tb.MX), tab<V>(tb.Ts),
+                         tab<V>(tb.To), tab<V>(tb.D1), tab<V>(tb.D1s), up(hx), up(hy)};
+    }
+    int64_t lin(int i1, int i2, int i3) const { return i1 + (int64_t)g.n1 * (i2 + (int64_t)g.n23 * i3); }
+};
+
+namespace {
+
+// Map between the reference's full-space vector and the merged interior grid.
+// merged (m, y, z, c): plane M = m + 1; M < n1-1 on g-, M > n1-1 on g+, M == n1-1 the face.
+void reduce_full(const eqd_op& op, const double* full, double* red /* NN*3, node-major comp */, bool check) {
+    const int n1 = op.g.n1, NX = op.tb.NX, NY = op.tb.NY;
+    const int64_t N = op.N;
+    double scale = 0.0;
+    if (check)
+        for (int64_t i = 0; i < op.ntot; ++i) scale = std::max(scale, std::fabs(full[i]));
+    for (int z = 0; z < NY; ++z)
+        for (int y = 0; y < NY; ++y)
+            for (int m = 0; m < NX; ++m) {
+                const int M = m + 1;
+                const int64_t node = m + (int64_t)NX * (y + (int64_t)NY * z);
+                for (int c = 0; c < 3; ++c) {
+                    double v;
+                    if (M < n1 - 1) v = full[c * N + op.lin(M, y + 1, z + 1)];
+                    else if (M > n1 - 1) v = full[3 * N + c * N + op.lin(M - (n1 - 1), y + 1, z + 1)];
+                    else {
+                        double a = full[c * N + op.lin(n1 - 1, y + 1, z + 1)];
+                        double b = full[3 * N + c * N + op.lin(0, y + 1, z + 1)];
+                        if (check && std::fabs(a - b) > 1e-12 * scale)
+                            throw Error(EQD_ERR_RANGE, "rhs is not in range(P): fault pair values differ");
+                        v = 0.5 * (a + b);
+                    }
+                    red[node * 3 + c] = v;
+                }
+            }
+    if (check) {  // far-field entries must vanish
+        const int n2 = op.g.n23;
+        for (int side = 0; side < 2; ++side)
+            for (int i3 = 0; i3 < n2; ++i3)
+                for (int i2 = 0; i2 < n2; ++i2)
+                    for (int i1 = 0; i1 < n1; ++i1) {
+                        bool ff = i2 == 0 || i2 == n2 - 1 || i3 == 0 || i3 == n2 - 1 || (side == 0 ? i1 == 0 : i1 == n1 - 1);
+                        if (!ff) continue;
+                        for (int c = 0; c < 3; ++c)
+                            if (std::fabs(full[side * 3 * N + c * N + op.lin(i1, i2, i3)]) > 1e-12 * scale)
+                                throw Error(EQD_ERR_RANGE, "rhs is not in range(P): far-field entries are nonzero");
+                    }
+    }
+}
+
+void embed_full(const eqd_op& op, const double* red, double* full) {
+    const int n1 = op.g.n1, NX = op.tb.NX, NY = op.tb.NY;
+    const int64_t N = op.N;
+    std::fill(full, full + op.ntot, 0.0);
+    for (int z = 0; z < NY; ++z)
+        for (int y = 0; y < NY; ++y)
+            for (int m = 0; m < NX; ++m) {
+                const int M = m + 1;
+                const int64_t node = m + (int64_t)NX * (y + (int64_t)NY * z);
+                for (int c = 0; c < 3; ++c) {
+                    const double v = red[node * 3 + c];
+                    if (M <= n1 - 1) full[c * N + op.lin(M, y + 1, z + 1)] = v;
+                    if (M >= n1 - 1) full[3 * N + c * N + op.lin(M - (n1 - 1), y + 1, z + 1)] = v;
+                }
+            }
+}
+
+// Slip vectors on Omega_f -> n23 x n23 face fields (two directions).
+void slip_fields(const eqd_op& op, const double* s2, const double* s3, double* f) {
+    const int n23 = op.g.n23;
+    std::fill(f, f + 2 * (size_t)n23 * n23, 0.0);
+    for (int k = 0; k < op.nf; ++k) {
+        const int iy = op.i2lo + k % op.n2f, iz = op.i2lo + k / op.n2f;
+        f[iy + (size_t)n23 * iz] = s2[k];
+        f[(size_t)n23 * n23 + iy + (size_t)n23 * iz] = s3[k];
+    }
+}
+
+}  // namespace
+
+// Per-solver device workspace for one batch shape.
+struct Workspace {
+    int K = 0, ntiles = 0, prec = -1;
+    size_t nvec = 0;
+    DevMem b, x, r, p, q, xc, fields, partial, cs, tile_on, tile_rel, colid, outpos;
+    int* h_tile_on = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int exec_iters = 0;
+    ~Workspace() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (h_tile_on) (void)hipHostFree(h_tile_on);
+    }
+};
+
+// Per-kernel event timing for measurement runs (kernel ids as eqd_solver_kernel_timing).
+struct KTimer {
+    static constexpr int kKinds = 6;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, size_t>> pending;  // kernel id, index of its begin event
+    size_t used = 0;
+    int cur_active = 0;
+    double ms[kKinds] = {};
+    int64_t launches[kKinds] = {}, active[kKinds] = {};
+    hipEvent_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void begin(int id, hipStream_t st) {
+        pending.push_back({id, used});
+        HIPCHK(hipEventRecord(next(), st));
+    }
+    void end(int, hipStream_t st) { HIPCHK(hipEventRecord(next(), st)); }
+    // after a stream sync: fold the pending pairs into the totals; returns their sum (ms)
+    double flush() {
+        double tot = 0;
+        for (auto& pr : pending) {
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, pool[pr.second], pool[pr.second + 1]));
+            ms[pr.first] += t;
+            launches[pr.first] += 1;
+            active[pr.first] += cur_active;
+            tot += t;
+        }
+        pending.clear();
+        used = 0;
+        return tot;
+    }
+    void reset() {
+        for (int i = 0; i < kKinds; ++i) ms[i] = 0, launches[i] = active[i] = 0;
+    }
+    ~KTimer() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+struct eqd_solver {
+    std::unique_ptr<KTimer> kt{new KTimer()};
+    eqd_op* op = nullptr;
+    eqd_cg_opts o{};
+    eqd_report st{};
+    hipStream_t stream = nullptr;
+    std::unique_ptr<Workspace> ws;
+    double last_ms = 0;
+    int64_t last_launches = 0, last_tile_iters = 0;
+    std::vector<int> last_iters;  // per column of the last batch (slot order)
+};
+
+namespace {
+
+size_t bytes_per_column(const eqd_op& op, int prec) {
+    const size_t n = (size_t)op.NN * 3;
+    const size_t t = prec == EQD_PREC_MIXED ? 4 : 8;
+    return n * (8 + 8 + 3 * t + (prec == EQD_PREC_MIXED ? 4 : 0)) + 2 * (size_t)op.g.n23 * op.g.n23 * 8;
+}
+
+void ensure_ws(eqd_solver* s, int K, int ntiles, int prec) {
+    Workspace* w = s->ws.get();
+    if (w && w->K == K && w->ntiles == ntiles && w->prec == prec) return;
+    s->ws.reset();
+    s->ws.reset(new Workspace());
+    w = s->ws.get();
+    const eqd_op& op = *s->op;
+    w->K = K;
+    w->ntiles = ntiles;
+    w->prec = prec;
+    w->nvec = (size_t)ntiles * op.NN * 3 * K;
+    const size_t slots = (size_t)ntiles * K;
+    const size_t t = prec == EQD_PREC_MIXED ? sizeof(float) : sizeof(double);
+    w->b.alloc(w->nvec * 8);
+    w->x.alloc(w->nvec * 8);
+    w->r.alloc(w->nvec * t);
+    w->p.alloc(w->nvec * t);
+    w->q.alloc(w->nvec * t);
+    if (prec == EQD_PREC_MIXED) w->xc.alloc(w->nvec * sizeof(float));
+    w->fields.alloc(slots * 2 * op.g.n23 * op.g.n23 * 8);
+    w->partial.alloc((size_t)ntiles * kGroups * K * 8);
+    w->cs.alloc(slots * sizeof(ColState));
+    w->tile_on.alloc(ntiles * sizeof(int));
+    w->tile_rel.alloc(ntiles * sizeof(int));
+    w->colid.alloc(slots * 8);
+    w->outpos.alloc(slots * 8);
+    HIPCHK(hipHostMalloc((void**)&w->h_tile_on, ntiles * sizeof(int), hipHostMallocDefault));
+}
+
+template <typename T, int K>
+struct Engine {
+    // One CG iteration for every tile in flight (device-side alpha/beta). With a
+    // timer, every launch is bracketed by events (measurement runs, no graph).
+    static void iteration(eqd_solver* s, hipStream_t st, KTimer* kt = nullptr) {
+        Workspace& w = *s->ws;
+        const eqd_op& op = *s->op;
+        const bool mixed = std::is_same<T, float>::value;
+        const int R = s->o.reliable_interval > 0 ? s->o.reliable_interval : 16;
+        dim3 grid(kGroups, w.ntiles);
+        T* r = w.r.as<T>();
+        T* p = w.p.as<T>();
+        T* q = w.q.as<T>();
+        ColState* cs = w.cs.as<ColState>();
+        int* on = w.tile_on.as<int>();
+        int* rel = w.tile_rel.as<int>();
+        double* part = w.partial.as<double>();
+        OpView<T> v;
+        if constexpr (std::is_same<T, float>::value) v = op.v32; else v = op.v64;
+        auto timed = [&](int id, auto&& launch) {
+            if (kt) kt->begin(id, st);
+            launch();
+            if (kt) kt->end(id, st);
+        };
+        timed(0, [&] { hipLaunchKernelGGL((k_apply_dot<T, K>), grid, dim3(kBlock), 0, st, v, op.NN, p, q, on, part); });
+        timed(3, [&] {
+            hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, 0, (int)mixed, R, cs, part, on, rel,
+                               nullptr);
+        });
+        timed(1, [&] {
+            if constexpr (std::is_same<T, float>::value)
+                hipLaunchKernelGGL((k_update<T, float, K>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface,
+                                   w.xc.as<float>(), r, p, q, cs, on, part);
+            else
+                hipLaunchKernelGGL((k_update<T, double, K>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface,
+                                   w.x.as<double>(), r, p, q, cs, on, part);
+        });
+        timed(3, [&] {
+            hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, 1, (int)mixed, R, cs, part, on, rel,
+                               nullptr);
+        });
+        timed(2, [&] { hipLaunchKernelGGL((k_pupdate<T, K>), grid, dim3(kBlock), 0, st, op.NN, p, r, cs, on); });
+        if (mixed) {
+            timed(4, [&] {
+                hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, op.NN, w.x.as<double>(), w.xc.as<float>(),
+                                   cs, rel, 0);
+            });
+            timed(5, [&] {
+                hipLaunchKernelGGL((k_residual<T, K>), grid, dim3(kBlock), 0, st, op.v64, op.NN, w.x.as<double>(),
+                                   w.b.as<double>(), r, cs, rel, part);
+            });
+            timed(3, [&] {
+                hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, 2, 1, R, cs, part, on, rel,
+                                   nullptr);
+            });
+        }
+        HIPCHK(hipGetLastError());
+    }
+
+    // Initial state once b, x, r, p (and xc) hold the start of every column.
+    static void init_state(eqd_solver* s, hipStream_t st) {
+        Workspace& w = *s->ws;
+        const eqd_op& op = *s->op;
+        dim3 grid(kGroups, w.ntiles);
+        hipLaunchKernelGGL((k_selfdot<double, K>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface,
+                           w.b.as<double>(), w.partial.as<double>());
+        const int64_t itmax = s->o.itmax > 0 ? s->o.itmax : 2 * op.ntot;
+        hipLaunchKernelGGL((k_init_state<K>), dim3(w.ntiles), dim3(K), 0, st, w.cs.as<ColState>(),
+                           w.partial.as<double>(), w.colid.as<int64_t>(), s->o.rtol, s->o.atol, itmax,
+                           w.tile_on.as<int>());
+        HIPCHK(hipGetLastError());
+    }
+
+    // Measurement run: no graph, events around every kernel, tile activity read
+    // before each iteration so the stencil's active tiles are known per launch.
+    static void solve_timed(eqd_solver* s) {
+        Workspace& w = *s->ws;
+        hipStream_t st = s->stream;
+        KTimer& kt = *s->kt;
+        const int64_t itmax = s->o.itmax > 0 ? s->o.itmax : 2 * s->op->ntot;
+        int64_t iters = 0, tile_iters = 0;
+        double ms = 0;
+        for (;;) {
+            HIPCHK(hipMemcpyAsync(w.h_tile_on, w.tile_on.p, w.ntiles * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            int act = 0;
+            for (int t = 0; t < w.ntiles; ++t) act += w.h_tile_on[t] != 0;
+            if (!act) break;
+            if (iters > itmax + 1) throw Error(EQD_ERR_DEVICE, "CG iteration guard exceeded");
+            kt.cur_active = act;
+            iteration(s, st, &kt);
+            HIPCHK(hipStreamSynchronize(st));
+            ms += kt.flush();
+            ++iters;
+            tile_iters += act;
+        }
+        if (std::is_same<T, float>::value) {
+            dim3 grid(kGroups, w.ntiles);
+            hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, s->op->NN, w.x.as<double>(),
+                               w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        s->last_ms = ms;
+        s->last_launches = iters;
+        s->last_tile_iters = tile_iters;
+    }
+
+    // Iterate until no tile has an active column; graphs of `gi` iterations.
+    static void solve(eqd_solver* s) {
+        if (s->o.timing) return solve_timed(s);
+        Workspace& w = *s->ws;
+        hipStream_t st = s->stream;
+        const int gi = s->o.graph_iters > 0 ? s->o.graph_iters : 8;
+        if (!w.exec || w.exec_iters != gi) {
+            if (w.exec) HIPCHK(hipGraphExecDestroy(w.exec));
+            w.exec = nullptr;
+            hipGraph_t graph;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < gi; ++i) iteration(s, st);
+            HIPCHK(hipStreamEndCapture(st, &graph));
+            HIPCHK(hipGraphInstantiate(&w.exec, graph, nullptr, nullptr, 0));
+            HIPCHK(hipGraphDestroy(graph));
+            w.exec_iters = gi;
+        }
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, st));
+        const int64_t itmax = s->o.itmax > 0 ? s->o.itmax : 2 * s->op->ntot;
+        int64_t replays = 0, tile_iters = 0;
+        for (;;) {
+            HIPCHK(hipMemcpyAsync(w.h_tile_on, w.tile_on.p, w.ntiles * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            int act = 0;
+            for (int t = 0; t < w.ntiles; ++t) act += w.h_tile_on[t] != 0;
+            if (!act) break;
+            if (replays * gi > itmax + gi) throw Error(EQD_ERR_DEVICE, "CG iteration guard exceeded");
+            HIPCHK(hipGraphLaunch(w.exec, st));
+            ++replays;
+            tile_iters += (int64_t)act * gi;
+        }
+        if (std::is_same<T, float>::value) {  // fold any correction still held in FP32
+            dim3 grid(kGroups, w.ntiles);
+            hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, s->op->NN, w.x.as<double>(),
+                               w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        s->last_ms = ms;
+        s->last_launches = replays * gi;
+        s->last_tile_iters = tile_iters;
+    }
+
+    // Plain engine: one column, host-side alpha/beta (device->host read of the partial
+    // sums every half iteration), no graphs -- the one-RHS baseline.
+    static void solve_plain(eqd_solver* s) {
+        static_assert(K == 1, "plain engine is single-column");
+        Workspace& w = *s->ws;
+        const eqd_op& op = *s->op;
+        hipStream_t st = s->stream;
+        dim3 grid(kGroups, 1);
+        std::vector<double> part(kGroups);
+        ColState h;
+        HIPCHK(hipMemcpy(&h, w.cs.p, sizeof(ColState), hipMemcpyDeviceToHost));
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, st));
+        const double meps = 2.220446049250313e-16;
+        int one = 1;
+        int64_t launches = 0;
+        HIPCHK(hipMemcpy(w.tile_on.p, &one, sizeof(int), hipMemcpyHostToDevice));
+        auto sum = [&]() {
+            HIPCHK(hipMemcpyAsync(part.data(), w.partial.p, kGroups * 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            double acc = 0.0;
+            for (double v : part) acc += v;
+            return acc;
+        };
+        while (h.status == kActive) {
+            hipLaunchKernelGGL((k_apply_dot<double, 1>), grid, dim3(kBlock), 0, st, op.v64, op.NN,
+                               w.p.as<double>(), w.q.as<double>(), w.tile_on.as<int>(), w.partial.as<double>());
+            ++launches;
+            const double pq = sum();
+            if (pq <= meps * h.pnorm2 && std::fabs(pq) <= meps * h.pnorm2) {
+                h.status = kZeroCurv;
+                break;
+            }
+            h.alpha = h.rr / pq;
+            HIPCHK(hipMemcpyAsync(w.cs.p, &h, sizeof(ColState), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL((k_update<double, double, 1>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface,
+                               w.x.as<double>(), w.r.as<double>(), w.p.as<double>(), w.q.as<double>(),
+                               w.cs.as<ColState>(), w.tile_on.as<int>(), w.partial.as<double>());
+            const double rr = sum();
+            const double rnorm = std::sqrt(rr);
+            h.iters += 1;
+            if (rnorm <= h.eps || rnorm + 1.0 <= 1.0) {
+                h.status = kSolved;
+                break;
+            }
+            h.beta = rr / h.rr;
+            h.pnorm2 = rr + h.beta * h.beta * h.pnorm2;
+            h.rr = rr;
+            if (h.iters >= h.itmax) {
+                h.status = kTired;
+                break;
+            }
+            HIPCHK(hipMemcpyAsync(w.cs.p, &h, sizeof(ColState), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL((k_pupdate<double, 1>), grid, dim3(kBlock), 0, st, op.NN, w.p.as<double>(),
+                               w.r.as<double>(), w.cs.as<ColState>(), w.tile_on.as<int>());
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipMemcpyAsync(w.cs.p, &h, sizeof(ColState), hipMemcpyHostToDevice, st));
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        s->last_ms = ms;
+        s->last_launches = launches;
+        s->last_tile_iters = h.iters;
+    }
+
+    // Builds b (and the Krylov start vectors) from the slip fields already on the device.
+    static void rhs_from_fields(eqd_solver* s) {
+        Workspace& w = *s->ws;
+        const eqd_op& op = *s->op;
+        dim3 grid(kGroups, w.ntiles);
+        hipLaunchKernelGGL((k_rhs<T, K>), grid, dim3(kBlock), 0, s->stream, op.rv, op.NN, w.fields.as<double>(),
+                           w.ntiles * K, w.b.as<double>(), w.x.as<double>(), w.r.as<T>(), w.p.as<T>(),
+                           std::is_same<T, float>::value ? w.xc.as<float>() : nullptr);
+        HIPCHK(hipGetLastError());
+    }
+
+    static void extract(eqd_solver* s, double* Kdev, int64_t ldK) {
+        Workspace& w = *s->ws;
+        const eqd_op& op = *s->op;
+        const int npb = kBlock / K;
+        dim3 grid((op.nf + npb - 1) / npb, w.ntiles);
+        hipLaunchKernelGGL((k_extract<K>), grid, dim3(kBlock), 0, s->stream, op.ev, op.NN, w.x.as<double>(),
+                           w.fields.as<double>(), w.outpos.as<int64_t>(), Kdev, ldK, (int64_t)0);
+        HIPCHK(hipGetLastError());
+    }
+};
+
+template <int K>
+struct Dispatch {
+    static void solve(eqd_solver* s) {
+        if (s->o.engine == EQD_ENGINE_PLAIN) {
+            if constexpr (K == 1) Engine<double, 1>::solve_plain(s);
+            else throw Error(EQD_ERR_ARG, "the plain engine runs one column (tile_k = 1)");
+        } else if (s->ws->prec == EQD_PREC_MIXED) Engine<float, K>::solve(s);
+        else Engine<double, K>::solve(s);
+    }
+    static void init(eqd_solver* s) {
+        if (s->ws->prec == EQD_PREC_MIXED) Engine<float, K>::init_state(s, s->stream);
+        else Engine<double, K>::init_state(s, s->stream);
+    }
+    static void rhs(eqd_solver* s) {
+        if (s->ws->prec == EQD_PREC_MIXED) Engine<float, K>::rhs_from_fields(s);
+        else Engine<double, K>::rhs_from_fields(s);
+    }
+    static void extract(eqd_solver* s, double* Kdev, int64_t ldK) { Engine<double, K>::extract(s, Kdev, ldK); }
+};
+
+template <typename F>
+void with_k(int K, F&& f) {
+    switch (K) {
+        case 1: f(Dispatch<1>()); break;
+        case 8: f(Dispatch<8>()); break;
+        case 16: f(Dispatch<16>()); break;
+        case 32: f(Dispatch<32>()); break;
+        case 64: f(Dispatch<64>()); break;
+        default: throw Error(EQD_ERR_ARG, "tile_k must be 1, 8, 16, 32 or 64");
+    }
+}
+
+int effective_k(const eqd_solver* s, int ncols) {
+    if (s->o.engine == EQD_ENGINE_PLAIN) return 1;
+    int K = s->o.tile_k;
+    if (K == 0) K = ncols >= 64 ? 64 : (ncols >= 16 ? 16 : 1);
+    return K;
+}
+
+// Solves a batch of columns whose slip fields are produced by `fill` into
+// ws->fields (host or device), then extracts tractions into Kdev[:, outpos].
+template <typename Fill>
+void run_columns(eqd_solver* s, const std::vector<int64_t>& colid, const std::vector<int64_t>& outpos, Fill&& fill,
+                 double* Kdev, int64_t ldK) {
+    const int n = (int)colid.size();
+    const int K = effective_k(s, n);
+    const int prec = s->o.engine == EQD_ENGINE_PLAIN ? EQD_PREC_FP64 : s->o.precision;
+    int per = s->o.max_cols_in_flight > 0 ? s->o.max_cols_in_flight : n;
+    {  // bound by device memory
+        size_t freeb = 0, totb = 0;
+        HIPCHK(hipMemGetInfo(&freeb, &totb));
+        size_t cap = (size_t)(0.6 * (double)freeb / (double)bytes_per_column(*s->op, prec));
+        per = (int)std::max<size_t>(1, std::min<size_t>((size_t)per, cap));
+    }
+    if (s->o.engine == EQD_ENGINE_PLAIN) per = 1;
+    const int ntiles = (std::min(per, n) + K - 1) / K;
+    ensure_ws(s, K, ntiles, prec);
+    Workspace& w = *s->ws;
+    const int slots = ntiles * K;
+    s->last_iters.assign(n, 0);
+    s->kt->reset();
+    double ms = 0;
+    int64_t launches = 0, tile_iters = 0;
+    for (int base = 0; base < n; base += slots) {
+        std::vector<int64_t> cid(slots, -1), opos(slots, -1);
+        for (int i = 0; i < slots && base + i < n; ++i) {
+            cid[i] = colid[base + i];
+            opos[i] = outpos[base + i];
+        }
+        HIPCHK(hipMemcpyAsync(w.colid.p, cid.data(), slots * 8, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(w.outpos.p, opos.data(), slots * 8, hipMemcpyHostToDevice, s->stream));
+        fill(w, base, slots);
+        with_k(K, [&](auto d) {
+            decltype(d)::rhs(s);
+            decltype(d)::init(s);
+            decltype(d)::solve(s);
+            decltype(d)::extract(s, Kdev, ldK);
+        });
+        HIPCHK(hipStreamSynchronize(s->stream));
+        ms += s->last_ms;
+        launches += s->last_launches;
+        tile_iters += s->last_tile_iters;
+        std::vector<ColState> hs(slots);
+        HIPCHK(hipMemcpy(hs.data(), w.cs.p, slots * sizeof(ColState), hipMemcpyDeviceToHost));
+        for (int i = 0; i < slots && base + i < n; ++i) {
+            const ColState& c = hs[i];
+            s->st.solves += 1;
+            s->st.iterations += c.iters;
+            s->st.max_iterations = std::max<int64_t>(s->st.max_iterations, c.iters);
+            s->last_iters[base + i] = c.iters;
+            if (c.status != kSolved) {
+                s->st.unconverged += 1;
+                if (s->st.unconverged == 1)
+                    std::fprintf(stderr,
+                                 "warning: CG did not converge on a split-node solve after %d iterations; "
+                                 "raise itmax or loosen rtol\n",
+                                 c.iters);
+            }
+        }
+    }
+    s->last_ms = ms;
+    s->last_launches = launches;
+    s->last_tile_iters = tile_iters;
+}
+
+void unit_fill(eqd_solver* s, Workspace& w, int slots) {
+    const eqd_op& op = *s->op;
+    hipLaunchKernelGGL(k_unit_fields, dim3(slots), dim3(256), 0, s->stream, op.g.n23, op.i2lo, op.n2f, op.nf, slots,
+                       w.colid.as<int64_t>(), w.fields.as<double>());
+    HIPCHK(hipGetLastError());
+}
+
+__global__ void k_toeplitz_fill(int nf, int n2, int n3, const double* __restrict__ ker, const int* __restrict__ have,
+                                double* __restrict__ K, int64_t ldK) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)nf * nf) return;
+    const int i = (int)(idx % nf), j = (int)(idx / nf);
+    const int ai = i % n2, bi = i / n2, aj = j % n2, bj = j / n2;
+    const int W2 = 2 * n2 - 1;
+    const int key = (ai - aj + n2 - 1) + W2 * (bi - bj + n3 - 1);
+    double v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+    if (have[key]) {
+        v0 = ker[4 * key];
+        v1 = ker[4 * key + 1];
+        v2 = ker[4 * key + 2];
+        v3 = ker[4 * key + 3];
+    }
+    K[i + ldK * j] = v0;
+    K[nf + i + ldK * j] = v1;
+    K[i + ldK * (nf + j)] = v2;
+    K[nf + i + ldK * (nf + j)] = v3;
+}
+
+void stiffness(eqd_solver* s, int mode, const int* cols, int ncols, double* Kdev, int64_t ldK) {
+    const eqd_op& op = *s->op;
+    const int nf = op.nf;
+    if (mode == EQD_STIFFNESS_EXACT) {
+        std::vector<int64_t> cid, opos;
+        if (!cols) {
+            if (ldK < 2 * nf) throw Error(EQD_ERR_ARG, "ldK must be >= 2nf");
+            for (int i = 0; i < 2 * nf; ++i) {
+                cid.push_back(i);
+                opos.push_back(i);
+            }
+        } else {
+            if (ncols < 0) throw Error(EQD_ERR_ARG, "ncols must be >= 0");
+            if (ldK < 2 * nf) throw Error(EQD_ERR_ARG, "ldK must be >= 2nf");
+            for (int i = 0; i < ncols; ++i) {
+                if (cols[i] < 0 || cols[i] >= 2 * nf) throw Error(EQD_ERR_ARG, "column index out of range");
+                cid.push_back(cols[i]);
+                opos.push_back(i);
+            }
+        }
+        if (cid.empty()) return;
+        run_columns(s, cid, opos, [&](Workspace& w, int, int slots) { unit_fill(s, w, slots); }, Kdev, ldK);
+        return;
+    }
+    if (mode != EQD_STIFFNESS_TOEPLITZ) throw Error(EQD_ERR_ARG, "mode must be EXACT or TOEPLITZ");
+    if (cols) throw Error(EQD_ERR_ARG, "the Toeplitz build produces the whole K (cols must be NULL)");
+    const int n2 = op.n2f, n3 = op.n2f;
+    if (nf != n2 * n3) throw Error(EQD_ERR_GRID, "the Toeplitz expansion assumes Omega_f forms a full n2 x n3 grid");
+    if (ldK < 2 * nf) throw Error(EQD_ERR_ARG, "ldK must be >= 2nf");
+    // sources in priority order: centre, then the four corners (src/FaultResponse.jl:351-353)
+    const int ac = (n2 + 1) / 2 - 1, bc = (n3 + 1) / 2 - 1;
+    const int src[5] = {ac + n2 * bc, 0, n2 - 1, n2 * (n3 - 1), n2 - 1 + n2 * (n3 - 1)};
+    std::vector<int64_t> cid, opos;
+    for (int k = 0; k < 5; ++k) {
+        cid.push_back(src[k]);
+        opos.push_back(2 * k);
+        cid.push_back(nf + src[k]);
+        opos.push_back(2 * k + 1);
+    }
+    DevMem cols10((size_t)2 * nf * 10 * 8);
+    run_columns(s, cid, opos, [&](Workspace& w, int, int slots) { unit_fill(s, w, slots); }, cols10.as<double>(),
+                2 * nf);
+    std::vector<double> hc((size_t)2 * nf * 10);
+    HIPCHK(hipMemcpy(hc.data(), cols10.p, hc.size() * 8, hipMemcpyDeviceToHost));
+    const int W2 = 2 * n2 - 1, W3 = 2 * n3 - 1;
+    std::vector<int> have((size_t)W2 * W3, 0);
+    std::vector<double> ker((size_t)W2 * W3 * 4, 0.0);
+    for (int k = 0; k < 5; ++k) {
+        const int as = src[k] % n2, bs = src[k] / n2;
+        const double* c2 = &hc[(size_t)(2 * k) * 2 * nf];
+        const double* c3 = &hc[(size_t)(2 * k + 1) * 2 * nf];
+        for (int i = 0; i < nf; ++i) {
+            const int ai = i % n2, bi = i / n2;
+            const size_t key = (size_t)(ai - as + n2 - 1) + (size_t)W2 * (bi - bs + n3 - 1);
+            if (have[key]) continue;  // the earlier source wins
+            have[key] = 1;
+            ker[4 * key] = c2[i];
+            ker[4 * key + 1] = c2[nf + i];
+            ker[4 * key + 2] = c3[i];
+            ker[4 * key + 3] = c3[nf + i];
+        }
+    }
+    DevMem dker(ker.size() * 8), dhave(have.size() * sizeof(int));
+    HIPCHK(hipMemcpy(dker.p, ker.data(), ker.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dhave.p, have.data(), have.size() * sizeof(int), hipMemcpyHostToDevice));
+    const int64_t tot = (int64_t)nf * nf;
+    hipLaunchKernelGGL(k_toeplitz_fill, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s->stream, nf, n2, n3,
+                       dker.as<double>(), dhave.as<int>(), Kdev, ldK);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* eqd_version(void) { return "eqd 0.1 (gfx950)"; }
+
+int64_t eqd_tables_export(const eqd_grid_desc* g, int which, int* start, int* width, double* coef) {
+    int64_t n = -1;
+    eqd_status rc = guard([&] {
+        if (!g) throw Error(EQD_ERR_ARG, "null argument");
+        validate_grid(*g);
+        HostTables t = build_tables(g->n1, g->n23, g->L_normal, g->L_fault, g->lambda, g->mu);
+        const HostTab* tabs[9] = {&t.AXs, &t.AXo, &t.MX, &t.Ts, &t.To, &t.D1, &t.D1s, &t.D1full, &t.D1ext};
+        const std::vector<double>* vecs[5] = {&t.Hx, &t.Hy, &t.Jt, &t.Jn, &t.Dn};
+        if (which >= 0 && which < 9) {
+            const HostTab& h = *tabs[which];
+            n = h.nrows;
+            if (start) std::memcpy(start, h.start.data(), h.nrows * sizeof(int));
+            if (width) std::memcpy(width, h.width.data(), h.nrows * sizeof(int));
+            if (coef) std::memcpy(coef, h.c.data(), h.c.size() * sizeof(double));
+        } else if (which >= 100 && which < 105) {
+            const std::vector<double>& v = *vecs[which - 100];
+            n = (int64_t)v.size();
+            if (coef) std::memcpy(coef, v.data(), v.size() * sizeof(double));
+        } else if (which == 105) {
+            n = 1;
+            if (coef) coef[0] = t.Jext;
+        } else {
+            throw Error(EQD_ERR_ARG, "unknown table id");
+        }
+    });
+    return rc == EQD_OK ? n : -1;
+}
+
+size_t eqd_last_error(char* buf, size_t len) {
+    if (buf && len) std::snprintf(buf, len, "%s", g_err.c_str());
+    return g_err.size();
+}
+
+void eqd_cg_opts_default(eqd_cg_opts* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->rtol = 1e-10;
+    o->atol = 0.0;
+    o->itmax = 0;
+    o->precond = EQD_PRECOND_NONE;
+    o->precision = EQD_PREC_FP64;
+    o->engine = EQD_ENGINE_BATCHED;
+    o->tile_k = 0;
+    o->max_cols_in_flight = 0;
+    o->reliable_interval = 16;
+    o->graph_iters = 8;
+}
+
+eqd_status eqd_operator_create(const eqd_grid_desc* g, eqd_op** out) {
+    return guard([&] {
+        if (!g || !out) throw Error(EQD_ERR_ARG, "null argument");
+        *out = nullptr;
+        validate_grid(*g);
+        int lo = 0, cnt = 0;
+        omega_range(*g, lo, cnt);
+        check_device();
+        std::unique_ptr<eqd_op> op(new eqd_op());
+        op->g = *g;
+        op->tb = build_tables(g->n1, g->n23, g->L_normal, g->L_fault, g->lambda, g->mu);
+        op->NN = op->tb.NX * op->tb.NY * op->tb.NY;
+        op->N = (int64_t)g->n1 * g->n23 * g->n23;
+        op->ntot = 6 * op->N;
+        op->i2lo = lo;
+        op->n2f = cnt;
+        op->nf = cnt * cnt;
+        op->v64 = op->view<double>();
+        op->v32 = op->view<float>();
+        op->rv = RhsView{op->tb.NX, op->tb.NY, g->n23, lo, cnt, op->nf, op->up(op->tb.Jt), op->up(op->tb.Jn),
+                         op->v64.Hx, op->v64.Hy, op->tab<double>(op->tb.D1full)};
+        op->ev = ExtractView{op->tb.NX, op->tb.NY, op->tb.mface, g->n23, lo, cnt, op->nf, g->mu, op->tb.Jext,
+                             op->up(op->tb.Dn), op->tab<double>(op->tb.D1ext)};
+        *out = op.release();
+    });
+}
+
+eqd_status eqd_operator_destroy(eqd_op* op) {
+    return guard([&] {
+        if (op && --op->refs == 0) delete op;
+    });
+}
+
+eqd_status eqd_operator_info(const eqd_op* op, eqd_op_info* info) {
+    return guard([&] {
+        if (!op || !info) throw Error(EQD_ERR_ARG, "null argument");
+        info->nf = op->nf;
+        info->n2f = info->n3f = op->n2f;
+        info->ntot = op->ntot;
+        info->nreduced = (int64_t)op->NN * 3;
+        info->h1 = op->tb.h1;
+        info->h2 = op->tb.h2;
+    });
+}
+
+eqd_status eqd_fault_grid_axes(const eqd_op* op, double* x2f, double* x3f) {
+    return guard([&] {
+        if (!op || !x2f || !x3f) throw Error(EQD_ERR_ARG, "null argument");
+        for (int i = 0; i < op->n2f; ++i) x2f[i] = x3f[i] = -op->g.L_fault + (op->i2lo + i) * op->tb.h2;
+    });
+}
+
+eqd_status eqd_apply_A(eqd_op* op, const double* x, double* y) {
+    return guard([&] {
+        if (!op || !x || !y) throw Error(EQD_ERR_ARG, "null argument");
+        const size_t n = (size_t)op->NN * 3;
+        std::vector<double> xr(n), yr(n);
+        reduce_full(*op, x, xr.data(), false);  // P x
+        DevMem dx(n * 8), dy(n * 8), part((size_t)kGroups * 8);
+        int one = 1;
+        DevMem on(sizeof(int));
+        HIPCHK(hipMemcpy(on.p, &one, sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dx.p, xr.data(), n * 8, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL((k_apply_dot<double, 1>), dim3(kGroups, 1), dim3(kBlock), 0, 0, op->v64, op->NN,
+                           dx.as<double>(), dy.as<double>(), on.as<int>(), part.as<double>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(yr.data(), dy.p, n * 8, hipMemcpyDeviceToHost));
+        embed_full(*op, yr.data(), y);
+    });
+}
+
+eqd_status eqd_solver_create(eqd_op* op, const eqd_cg_opts* opts, eqd_solver** out) {
+    return guard([&] {
+        if (!op || !out) throw Error(EQD_ERR_ARG, "null argument");
+        *out = nullptr;
+        eqd_cg_opts o;
+        eqd_cg_opts_default(&o);
+        if (opts) o = *opts;
+        if (o.precond == EQD_PRECOND_JACOBI)
+            throw Error(EQD_ERR_ARG, "precond :jacobi is not provided on the device (it measured 0.92x in the reference)");
+        if (o.precond != EQD_PRECOND_NONE) throw Error(EQD_ERR_ARG, "precond must be :none or :jacobi");
+        if (o.precision != EQD_PREC_FP64 && o.precision != EQD_PREC_MIXED) throw Error(EQD_ERR_ARG, "bad precision");
+        if (o.engine != EQD_ENGINE_BATCHED && o.engine != EQD_ENGINE_PLAIN) throw Error(EQD_ERR_ARG, "bad engine");
+        if (!(o.rtol >= 0) || !(o.atol >= 0)) throw Error(EQD_ERR_ARG, "rtol and atol must be >= 0");
+        std::unique_ptr<eqd_solver> s(new eqd_solver());
+        s->op = op;
+        s->o = o;
+        HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        op->refs++;
+        *out = s.release();
+    });
+}
+
+eqd_status eqd_solver_destroy(eqd_solver* s) {
+    return guard([&] {
+        if (!s) return;
+        s->ws.reset();
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        eqd_op* op = s->op;
+        delete s;
+        if (op && --op->refs == 0) delete op;
+    });
+}
+
+eqd_status eqd_solver_duplicate(const eqd_solver* s, eqd_solver** out) {
+    return guard([&] {
+        if (!s || !out) throw Error(EQD_ERR_ARG, "null argument");
+        eqd_status rc = eqd_solver_create(s->op, &s->o, out);
+        if (rc) throw Error(rc, g_err);
+    });
+}
+
+eqd_status eqd_solver_merge_stats(eqd_solver* into, const eqd_solver* from) {
+    return guard([&] {
+        if (!into || !from) throw Error(EQD_ERR_ARG, "null argument");
+        into->st.solves += from->st.solves;
+        into->st.iterations += from->st.iterations;
+        into->st.max_iterations = std::max(into->st.max_iterations, from->st.max_iterations);
+        into->st.unconverged += from->st.unconverged;
+    });
+}
+
+eqd_status eqd_solver_report(const eqd_solver* s, eqd_report* r) {
+    return guard([&] {
+        if (!s || !r) throw Error(EQD_ERR_ARG, "null argument");
+        *r = s->st;
+        r->mean_iterations = s->st.solves ? (double)s->st.iterations / (double)s->st.solves : 0.0;
+        r->precond = s->o.precond;
+    });
+}
+
+eqd_status eqd_solver_last_timing(const eqd_solver* s, double* ms, int64_t* launches, int64_t* tile_iters) {
+    return guard([&] {
+        if (!s) throw Error(EQD_ERR_ARG, "null argument");
+        if (ms) *ms = s->last_ms;
+        if (launches) *launches = s->last_launches;
+        if (tile_iters) *tile_iters = s->last_tile_iters;
+    });
+}
+
+eqd_status eqd_solver_kernel_timing(const eqd_solver* s, int kernel, double* total_ms, int64_t* launches,
+                                    int64_t* active_tiles) {
+    return guard([&] {
+        if (!s) throw Error(EQD_ERR_ARG, "null argument");
+        if (kernel < 0 || kernel >= KTimer::kKinds) throw Error(EQD_ERR_ARG, "unknown kernel id");
+        if (total_ms) *total_ms = s->kt->ms[kernel];
+        if (launches) *launches = s->kt->launches[kernel];
+        if (active_tiles) *active_tiles = s->kt->active[kernel];
+    });
+}
+
+eqd_status eqd_build_rhs(eqd_op* op, const double* s2, const double* s3, double* rhs) {
+    return guard([&] {
+        if (!op || !s2 || !s3 || !rhs) throw Error(EQD_ERR_ARG, "null argument");
+        const int n23 = op->g.n23;
+        const size_t n = (size_t)op->NN * 3;
+        std::vector<double> f(2 * (size_t)n23 * n23), b(n);
+        slip_fields(*op, s2, s3, f.data());
+        DevMem df(f.size() * 8), db(n * 8), dx(n * 8), dr(n * 8), dp(n * 8);
+        HIPCHK(hipMemcpy(df.p, f.data(), f.size() * 8, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL((k_rhs<double, 1>), dim3(kGroups, 1), dim3(kBlock), 0, 0, op->rv, op->NN, df.as<double>(), 1,
+                           db.as<double>(), dx.as<double>(), dr.as<double>(), dp.as<double>(), (float*)nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(b.data(), db.p, n * 8, hipMemcpyDeviceToHost));
+        embed_full(*op, b.data(), rhs);
+    });
+}
+
+eqd_status eqd_split_node_solve(eqd_solver* s, const double* rhs, double* u) {
+    return guard([&] {
+        if (!s || !rhs || !u) throw Error(EQD_ERR_ARG, "null argument");
+        const eqd_op& op = *s->op;
+        const size_t n = (size_t)op.NN * 3;
+        std::vector<double> br(n);
+        reduce_full(op, rhs, br.data(), true);
+        const int K = s->o.engine == EQD_ENGINE_PLAIN ? 1 : 1;
+        const int prec = s->o.engine == EQD_ENGINE_PLAIN ? EQD_PREC_FP64 : s->o.precision;
+        ensure_ws(s, K, 1, prec);
+        Workspace& w = *s->ws;
+        int64_t cid = 0;
+        HIPCHK(hipMemcpy(w.colid.p, &cid, 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(w.b.p, br.data(), n * 8, hipMemcpyHostToDevice));
+        // x = 0, r = p = b
+        HIPCHK(hipMemset(w.x.p, 0, n * 8));
+        if (prec == EQD_PREC_MIXED) {
+            std::vector<float> bf(br.begin(), br.end());
+            HIPCHK(hipMemcpy(w.r.p, bf.data(), n * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(w.p.p, bf.data(), n * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemset(w.xc.p, 0, n * 4));
+        } else {
+            HIPCHK(hipMemcpy(w.r.p, br.data(), n * 8, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(w.p.p, br.data(), n * 8, hipMemcpyHostToDevice));
+        }
+        with_k(1, [&](auto d) {
+            decltype(d)::init(s);
+            decltype(d)::solve(s);
+        });
+        ColState c;
+        HIPCHK(hipMemcpy(&c, w.cs.p, sizeof(c), hipMemcpyDeviceToHost));
+        s->st.solves += 1;
+        s->st.iterations += c.iters;
+        s->st.max_iterations = std::max<int64_t>(s->st.max_iterations, c.iters);
+        if (c.status != kSolved) {
+            s->st.unconverged += 1;
+            if (s->st.unconverged == 1)
+                std::fprintf(stderr, "warning: CG did not converge on a split-node solve after %d iterations\n", c.iters);
+        }
+        std::vector<double> xr(n);
+        HIPCHK(hipMemcpy(xr.data(), w.x.p, n * 8, hipMemcpyDeviceToHost));
+        embed_full(op, xr.data(), u);
+    });
+}
+
+eqd_status eqd_shear_traction(eqd_solver* s, const double* s2, const double* s3, double* dtau2, double* dtau3) {
+    return guard([&] {
+        if (!s || !s2 || !s3 || !dtau2 || !dtau3) throw Error(EQD_ERR_ARG, "null argument");
+        const eqd_op& op = *s->op;
+        const int n23 = op.g.n23;
+        std::vector<double> f(2 * (size_t)n23 * n23);
+        slip_fields(op, s2, s3, f.data());
+        DevMem out((size_t)2 * op.nf * 8);
+        std::vector<int64_t> cid{0}, opos{0};
+        eqd_cg_opts saved = s->o;
+        if (s->o.engine != EQD_ENGINE_PLAIN) s->o.tile_k = 1;
+        try {
+            run_columns(s, cid, opos,
+                        [&](Workspace& w, int, int) {
+                            HIPCHK(hipMemcpy(w.fields.p, f.data(), f.size() * 8, hipMemcpyHostToDevice));
+                        },
+                        out.as<double>(), 2 * op.nf);
+        } catch (...) {
+            s->o = saved;
+            throw;
+        }
+        s->o = saved;
+        std::vector<double> h((size_t)2 * op.nf);
+        HIPCHK(hipMemcpy(h.data(), out.p, h.size() * 8, hipMemcpyDeviceToHost));
+        std::memcpy(dtau2, h.data(), op.nf * 8);
+        std::memcpy(dtau3, h.data() + op.nf, op.nf * 8);
+    });
+}
+
+eqd_status eqd_fault_stiffness_device(eqd_solver* s, int mode, const int* cols, int ncols, double* K_dev, int64_t ldK) {
+    return guard([&] {
+        if (!s || !K_dev) throw Error(EQD_ERR_ARG, "null argument");
+        stiffness(s, mode, cols, ncols, K_dev, ldK);
+    });
+}
+
+eqd_status eqd_fault_stiffness(eqd_solver* s, int mode, const int* cols, int ncols, double* K, int64_t ldK) {
+    return guard([&] {
+        if (!s || !K) throw Error(EQD_ERR_ARG, "null argument");
+        const int nf = s->op->nf;
+        const int64_t nout = (mode == EQD_STIFFNESS_EXACT && cols) ? ncols : 2 * (int64_t)nf;
+        if (ldK < 2 * nf) throw Error(EQD_ERR_ARG, "ldK must be >= 2nf");
+        DevMem dK((size_t)std::max<int64_t>(nout, 1) * ldK * 8);
+        stiffness(s, mode, cols, ncols, dK.as<double>(), ldK);
+        if (nout > 0) HIPCHK(hipMemcpy(K, dK.p, (size_t)nout * ldK * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"

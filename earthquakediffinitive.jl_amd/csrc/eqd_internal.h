@@ -1,0 +1,86 @@
+// Internal declarations of libeqd: operator tables, device views and kernels.
+//
+// Vector space. Every Krylov vector lives on range(P) of the reference's split-node
+// system (src/ElasticitySplitNode.jl:171-211): far-field DOFs are exactly zero and
+// the two DOFs of a fault pair are equal. The library therefore stores one merged
+// grid: x1 runs over both blocks with the fault face stored once, and only interior
+// nodes are kept.
+//   merged x index  m in [0, NX),  NX = 2*n1 - 3   (plane M = m + 1 of 2*n1 - 1)
+//   y, z index      in [0, NY),    NY = n23 - 2
+//   node            = m + NX*(y + NY*z)
+// A full-space dot product equals the merged one with weight 2 on the face plane.
+//
+// Batched layout ("tile-interleaved"): columns are grouped in tiles of K; element
+// (node, comp, column j of tile t) sits at ((t*NN + node)*3 + comp)*K + j, so the K
+// columns of one node are contiguous and a wavefront with K = 64 handles one node
+// for 64 right-hand sides with uniform control flow.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace eqd {
+
+constexpr int kTabW = 16;  // max taps of one 1D table row (zero padded)
+
+// One banded 1D operator, row r reads input columns start[r] .. start[r]+width[r]-1.
+struct HostTab {
+    int nrows = 0;
+    std::vector<int> start, width;
+    std::vector<double> c;  // nrows * kTabW
+};
+
+// All 1D data the kernels need, built once per operator on the host.
+struct HostTables {
+    int n1 = 0, n23 = 0, NX = 0, NY = 0, mface = 0;
+    double h1 = 0, h2 = 0, lambda = 0, mu = 0;
+    // reduced operator -H P (D+SAT) P on the merged interior grid
+    HostTab AXs, AXo, MX;        // x: same-comp (c=0), other-comp (c=1,2), (lam+mu) D1x
+    HostTab Ts, To, D1, D1s;     // y,z: same-comp, other-comp, D1, (lam+mu) D1
+    std::vector<double> Hx, Hy;  // quadrature weights (merged x, interior y)
+    // right-hand side H P (D+SAT) chi(s): jump coefficients along merged x
+    std::vector<double> Jt, Jn;  // tangential comps (from s directly), normal comp (from D1 s)
+    HostTab D1full;              // interior rows, full columns (slip may sit on the ring)
+    // traction extraction 1/2 (T U- + T U+) on Omega_f
+    std::vector<double> Dn;      // normal-derivative weights on merged x (times mu/2)
+    double Jext = 0;             // coefficient of the slip itself
+    HostTab D1ext;               // rows: full y index; cols: interior
+};
+
+HostTables build_tables(int n1, int n23, double L_normal, double L_fault, double lambda, double mu);
+
+// Device view of one table (typed coefficients).
+template <typename T>
+struct Tab {
+    const int* start;
+    const int* width;
+    const T* c;
+};
+
+template <typename T>
+struct OpView {
+    int NX, NY, mface;
+    Tab<T> AXs, AXo, MX, Ts, To, D1, D1s;
+    const T* Hx;
+    const T* Hy;
+};
+
+struct ExtractView {
+    int NX, NY, mface, n23, i2lo, n2f, nf;
+    double mu, Jext;
+    const double* Dn;
+    Tab<double> D1ext;
+};
+
+struct RhsView {
+    int NX, NY, n23, i2lo, n2f, nf;
+    const double* Jt;
+    const double* Jn;
+    const double* Hx;
+    const double* Hy;
+    Tab<double> D1full;
+};
+
+}  // namespace eqd

@@ -1,0 +1,519 @@
+// Device kernels of libeqd (gfx950). Included once, by eqd.hip.
+//
+// Batched layout: see eqd_internal.h. Grid convention for the node-parallel kernels:
+// blockIdx.y = tile, blockIdx.x = group g in [0, kGroups); a block of 256 threads is
+// (256/K) node slots x K columns and walks the tile's nodes with stride
+// kGroups*(256/K). Per-column partial sums are reduced in a fixed order (thread ->
+// block slot 0 -> group index), so a column's arithmetic does not depend on which
+// tile or lane holds it, nor on how many tiles are in flight.
+#pragma once
+#include "eqd_internal.h"
+
+namespace eqd {
+
+constexpr int kGroups = 64;
+constexpr int kBlock = 256;
+
+enum ColStatus : int { kActive = 0, kSolved = 1, kTired = 2, kZeroCurv = 3, kIdle = 4 };
+
+// Per-column CG state (one slot per column of every tile in flight).
+struct ColState {
+    double rr;      // current (weighted) r.r
+    double alpha, beta;
+    double eps;     // atol + rtol*||b||
+    double pnorm2;  // Krylov's running ||p||^2 estimate (zero-curvature test)
+    double bnorm;
+    int iters;
+    int status;
+    int since_rel;  // iterations since the last reliable update (mixed precision)
+    int rel_flag;   // 1: a reliable update is due after this iteration
+    int64_t itmax;
+    int64_t col;    // global column id (-1 = idle slot)
+};
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// q = -H (reduced split-node operator) p for the node (m, y, z) of one column.
+template <typename T, int K>
+__device__ __forceinline__ void stencil_node(const OpView<T>& op, const T* __restrict__ in, int m, int y, int z,
+                                             int lane, T out[3]) {
+    const int NX = op.NX, NY = op.NY;
+    auto ld = [&](int mm, int yy, int zz, int c) -> T {
+        return in[((size_t)(mm + NX * (yy + NY * zz)) * 3 + c) * K + lane];
+    };
+    T a0 = 0, a1 = 0, a2 = 0;
+    {  // x1 direction, same component (sigma_11 family) and the two others
+        const int s = op.AXs.start[m], w = op.AXs.width[m];
+        const T* c = op.AXs.c + (size_t)m * kTabW;
+        for (int t = 0; t < w; ++t) a0 += c[t] * ld(s + t, y, z, 0);
+        const int s2 = op.AXo.start[m], w2 = op.AXo.width[m];
+        const T* c2 = op.AXo.c + (size_t)m * kTabW;
+        for (int t = 0; t < w2; ++t) {
+            a1 += c2[t] * ld(s2 + t, y, z, 1);
+            a2 += c2[t] * ld(s2 + t, y, z, 2);
+        }
+    }
+    {  // x2 direction
+        const int s = op.To.start[y], w = op.To.width[y];
+        const T* c = op.To.c + (size_t)y * kTabW;
+        for (int t = 0; t < w; ++t) {
+            a0 += c[t] * ld(m, s + t, z, 0);
+            a2 += c[t] * ld(m, s + t, z, 2);
+        }
+        const int s2 = op.Ts.start[y], w2 = op.Ts.width[y];
+        const T* c2 = op.Ts.c + (size_t)y * kTabW;
+        for (int t = 0; t < w2; ++t) a1 += c2[t] * ld(m, s2 + t, z, 1);
+    }
+    {  // x3 direction
+        const int s = op.To.start[z], w = op.To.width[z];
+        const T* c = op.To.c + (size_t)z * kTabW;
+        for (int t = 0; t < w; ++t) {
+            a0 += c[t] * ld(m, y, s + t, 0);
+            a1 += c[t] * ld(m, y, s + t, 1);
+        }
+        const int s2 = op.Ts.start[z], w2 = op.Ts.width[z];
+        const T* c2 = op.Ts.c + (size_t)z * kTabW;
+        for (int t = 0; t < w2; ++t) a2 += c2[t] * ld(m, y, s2 + t, 2);
+    }
+    {  // mixed x1-x2: (lam+mu) D1x D1y
+        const int sx = op.MX.start[m], wx = op.MX.width[m];
+        const T* cx = op.MX.c + (size_t)m * kTabW;
+        const int sy = op.D1.start[y], wy = op.D1.width[y];
+        const T* cy = op.D1.c + (size_t)y * kTabW;
+        for (int tx = 0; tx < wx; ++tx) {
+            T s0 = 0, s1 = 0;
+            for (int ty = 0; ty < wy; ++ty) {
+                s0 += cy[ty] * ld(sx + tx, sy + ty, z, 1);
+                s1 += cy[ty] * ld(sx + tx, sy + ty, z, 0);
+            }
+            a0 += cx[tx] * s0;
+            a1 += cx[tx] * s1;
+        }
+    }
+    {  // mixed x1-x3
+        const int sx = op.MX.start[m], wx = op.MX.width[m];
+        const T* cx = op.MX.c + (size_t)m * kTabW;
+        const int sz = op.D1.start[z], wz = op.D1.width[z];
+        const T* cz = op.D1.c + (size_t)z * kTabW;
+        for (int tx = 0; tx < wx; ++tx) {
+            T s0 = 0, s2 = 0;
+            for (int tz = 0; tz < wz; ++tz) {
+                s0 += cz[tz] * ld(sx + tx, y, sz + tz, 2);
+                s2 += cz[tz] * ld(sx + tx, y, sz + tz, 0);
+            }
+            a0 += cx[tx] * s0;
+            a2 += cx[tx] * s2;
+        }
+    }
+    {  // mixed x2-x3
+        const int sy = op.D1s.start[y], wy = op.D1s.width[y];
+        const T* cy = op.D1s.c + (size_t)y * kTabW;
+        const int sz = op.D1.start[z], wz = op.D1.width[z];
+        const T* cz = op.D1.c + (size_t)z * kTabW;
+        for (int ty = 0; ty < wy; ++ty) {
+            T s1 = 0, s2 = 0;
+            for (int tz = 0; tz < wz; ++tz) {
+                s1 += cz[tz] * ld(m, sy + ty, sz + tz, 2);
+                s2 += cz[tz] * ld(m, sy + ty, sz + tz, 1);
+            }
+            a1 += cy[ty] * s1;
+            a2 += cy[ty] * s2;
+        }
+    }
+    const T hw = -(op.Hx[m] * op.Hy[y] * op.Hy[z]);
+    out[0] = hw * a0;
+    out[1] = hw * a1;
+    out[2] = hw * a2;
+}
+
+// Block-level fixed-order reduction of one double per thread into partial[(tile*G+g)*K+lane].
+template <int K>
+__device__ __forceinline__ void block_partial(double acc, double* __restrict__ partial, int tile, int g) {
+    constexpr int NPB = kBlock / K;
+    __shared__ double sm[kBlock];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double s = 0.0;
+        for (int i = 0; i < NPB; ++i) s += sm[i * K + threadIdx.x];
+        partial[((size_t)tile * kGroups + g) * K + threadIdx.x] = s;
+    }
+}
+
+// CG stencil pass: q = A p and the weighted partial dot p.q. Tiles with no active
+// column are skipped (their partials are written as zero).
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_apply_dot(OpView<T> op, int NN, const T* __restrict__ p,
+                                                      T* __restrict__ q, const int* __restrict__ tile_on,
+                                                      double* __restrict__ partial) {
+    constexpr int NPB = kBlock / K;
+    const int tile = blockIdx.y, g = blockIdx.x;
+    const int lane = threadIdx.x % K, slot = threadIdx.x / K;
+    double acc = 0.0;
+    if (tile_on[tile]) {
+        const T* pt = p + (size_t)tile * NN * 3 * K;
+        T* qt = q + (size_t)tile * NN * 3 * K;
+        for (int base = g * NPB; base < NN; base += kGroups * NPB) {
+            int node = base + slot;
+            if (K == 64) node = uni(node);
+            if (node >= NN) continue;
+            const int m = node % op.NX, yz = node / op.NX, y = yz % op.NY, z = yz / op.NY;
+            T o[3];
+            stencil_node<T, K>(op, pt, m, y, z, lane, o);
+            const size_t e = (size_t)node * 3 * K + lane;
+            const double w = (m == op.mface) ? 2.0 : 1.0;
+            double d = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                qt[e + c * K] = o[c];
+                d += (double)pt[e + c * K] * (double)o[c];
+            }
+            acc += w * d;
+        }
+    }
+    block_partial<K>(acc, partial, tile, g);
+}
+
+// Reliable-update residual in FP64: r = b - A x, stored to r (type T) for flagged
+// columns, with the weighted partial r.r taken on the FP64 values.
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_residual(OpView<double> op, int NN, const double* __restrict__ x,
+                                                     const double* __restrict__ b, T* __restrict__ r,
+                                                     const ColState* __restrict__ cs, const int* __restrict__ tile_on,
+                                                     double* __restrict__ partial) {
+    constexpr int NPB = kBlock / K;
+    const int tile = blockIdx.y, g = blockIdx.x;
+    const int lane = threadIdx.x % K, slot = threadIdx.x / K;
+    const bool mine = cs[(size_t)tile * K + lane].rel_flag != 0;
+    double acc = 0.0;
+    if (tile_on[tile]) {
+        const size_t off = (size_t)tile * NN * 3 * K;
+        for (int base = g * NPB; base < NN; base += kGroups * NPB) {
+            int node = base + slot;
+            if (K == 64) node = uni(node);
+            if (node >= NN) continue;
+            const int m = node % op.NX, yz = node / op.NX, y = yz % op.NY, z = yz / op.NY;
+            double o[3];
+            stencil_node<double, K>(op, x + off, m, y, z, lane, o);
+            const size_t e = off + (size_t)node * 3 * K + lane;
+            const double w = (m == op.mface) ? 2.0 : 1.0;
+            double d = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double rv = b[e + c * K] - o[c];
+                if (mine) r[e + c * K] = (T)rv;
+                d += rv * rv;
+            }
+            acc += w * d;
+        }
+    }
+    block_partial<K>(acc, partial, tile, g);
+}
+
+// x += alpha p; r -= alpha q; partial r.r   (X = solution accumulator type)
+template <typename T, typename X, int K>
+__global__ __launch_bounds__(kBlock) void k_update(int NN, int NX, int mface, X* __restrict__ x,
+                                                   T* __restrict__ r, const T* __restrict__ p,
+                                                   const T* __restrict__ q, const ColState* __restrict__ cs,
+                                                   const int* __restrict__ tile_on, double* __restrict__ partial) {
+    constexpr int NPB = kBlock / K;
+    const int tile = blockIdx.y, g = blockIdx.x;
+    const int lane = threadIdx.x % K, slot = threadIdx.x / K;
+    const ColState& st = cs[(size_t)tile * K + lane];
+    double acc = 0.0;
+    if (tile_on[tile] && st.status == kActive) {
+        const X a = (X)st.alpha;
+        const T at = (T)st.alpha;
+        const size_t off = (size_t)tile * NN * 3 * K;
+        for (int base = g * NPB; base < NN; base += kGroups * NPB) {
+            const int node = base + slot;
+            if (node >= NN) continue;
+            const size_t e = off + (size_t)node * 3 * K + lane;
+            const double w = (node % NX == mface) ? 2.0 : 1.0;
+            double d = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const size_t i = e + c * K;
+                x[i] += a * (X)p[i];
+                T rv = r[i] - at * q[i];
+                r[i] = rv;
+                d += (double)rv * (double)rv;
+            }
+            acc += w * d;
+        }
+    }
+    block_partial<K>(acc, partial, tile, g);
+}
+
+// p = r + beta p
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_pupdate(int NN, T* __restrict__ p, const T* __restrict__ r,
+                                                    const ColState* __restrict__ cs, const int* __restrict__ tile_on) {
+    constexpr int NPB = kBlock / K;
+    const int tile = blockIdx.y, g = blockIdx.x;
+    const int lane = threadIdx.x % K, slot = threadIdx.x / K;
+    const ColState& st = cs[(size_t)tile * K + lane];
+    if (!tile_on[tile] || st.status != kActive) return;
+    const T b = (T)st.beta;
+    const size_t off = (size_t)tile * NN * 3 * K;
+    for (int base = g * NPB; base < NN; base += kGroups * NPB) {
+        const int node = base + slot;
+        if (node >= NN) continue;
+        const size_t e = off + (size_t)node * 3 * K + lane;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[e + c * K] = r[e + c * K] + b * p[e + c * K];
+    }
+}
+
+// Reliable update, part 1: x64 += xc, xc = 0 for flagged columns (every column when all != 0).
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_xaccum(int NN, double* __restrict__ x, float* __restrict__ xc,
+                                                   const ColState* __restrict__ cs, const int* __restrict__ tile_on,
+                                                   int all) {
+    constexpr int NPB = kBlock / K;
+    const int tile = blockIdx.y, g = blockIdx.x;
+    const int lane = threadIdx.x % K, slot = threadIdx.x / K;
+    if (!all && (!tile_on[tile] || !cs[(size_t)tile * K + lane].rel_flag)) return;
+    const size_t off = (size_t)tile * NN * 3 * K;
+    for (int base = g * NPB; base < NN; base += kGroups * NPB) {
+        const int node = base + slot;
+        if (node >= NN) continue;
+        const size_t e = off + (size_t)node * 3 * K + lane;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            x[e + c * K] += (double)xc[e + c * K];
+            xc[e + c * K] = 0.0f;
+        }
+    }
+}
+
+template <int K>
+__device__ __forceinline__ double sum_partials(const double* __restrict__ partial, int tile, int lane) {
+    double s = 0.0;
+    for (int g = 0; g < kGroups; ++g) s += partial[((size_t)tile * kGroups + g) * K + lane];
+    return s;
+}
+
+// One block per tile, K threads: per-column scalar steps of Krylov's cg!
+// (src/ElasticitySplitNode.jl:379 -> Krylov 0.10.9) and the tile activity flags.
+// mode 0: after the stencil (alpha, zero-curvature exit)
+// mode 1: after the update (convergence test, beta, iteration count, itmax)
+// mode 2: after a reliable-update residual (mixed precision only)
+template <int K>
+__global__ void k_finalize(int mode, int mixed, int rel_interval, ColState* __restrict__ cs,
+                           const double* __restrict__ partial, int* __restrict__ tile_on,
+                           int* __restrict__ tile_rel, int* __restrict__ active_count) {
+    const int tile = blockIdx.x, lane = threadIdx.x;
+    ColState& st = cs[(size_t)tile * K + lane];
+    const double meps = 2.220446049250313e-16;
+    if (st.status == kActive) {
+        const double s = sum_partials<K>(partial, tile, lane);
+        if (mode == 0) {
+            if (s <= meps * st.pnorm2 && fabs(s) <= meps * st.pnorm2) {
+                st.status = kZeroCurv;
+            } else {
+                st.alpha = st.rr / s;
+            }
+        } else if (mode == 1) {
+            const double rnorm = sqrt(s);
+            const bool solved = rnorm <= st.eps || rnorm + 1.0 <= 1.0;
+            st.iters += 1;
+            st.since_rel += 1;
+            st.rel_flag = 0;
+            if (solved && !mixed) {
+                st.status = kSolved;
+            } else {
+                const double beta = s / st.rr;
+                st.beta = beta;
+                st.pnorm2 = s + beta * beta * st.pnorm2;
+                st.rr = s;
+                if (mixed && (solved || st.since_rel >= rel_interval)) st.rel_flag = 1;
+                if (st.iters >= st.itmax) {
+                    st.status = kTired;
+                    st.rel_flag = mixed ? 1 : 0;  // fold the last correction into x
+                }
+            }
+        } else {
+            if (st.rel_flag) {
+                const double rnorm = sqrt(s);
+                st.since_rel = 0;
+                if (rnorm <= st.eps || rnorm + 1.0 <= 1.0) st.status = kSolved;
+                else st.rr = s;
+            }
+        }
+    } else if (mode == 2 && st.rel_flag) {
+        st.since_rel = 0;  // a column that hit itmax in this iteration: its correction was folded
+    }
+    if (mode == 2) st.rel_flag = 0;
+    const int on = __syncthreads_or(st.status == kActive);
+    const int rel = __syncthreads_or(st.rel_flag != 0);
+    if (lane == 0) {
+        tile_on[tile] = on;
+        if (mode == 1) tile_rel[tile] = rel;
+        if (active_count && on) atomicAdd(active_count, 1);
+    }
+}
+
+// Unit-slip fields on the face grid (n23 x n23 per column, two directions):
+// column id < nf -> s2 = e_col, else s3 = e_(col - nf)   (src/FaultResponse.jl:249-253).
+__global__ void k_unit_fields(int n23, int i2lo, int n2f, int nf, int ncols_slots, const int64_t* __restrict__ colid,
+                              double* __restrict__ fields) {
+    const int slot = blockIdx.x;
+    if (slot >= ncols_slots) return;
+    double* f = fields + (size_t)slot * 2 * n23 * n23;
+    for (int i = threadIdx.x; i < 2 * n23 * n23; i += blockDim.x) f[i] = 0.0;
+    __syncthreads();
+    const int64_t c = colid[slot];
+    if (threadIdx.x == 0 && c >= 0) {
+        const int dir = c < nf ? 0 : 1;
+        const int k = (int)(dir == 0 ? c : c - nf);
+        const int iy = i2lo + k % n2f, iz = i2lo + k / n2f;
+        f[(size_t)dir * n23 * n23 + iy + (size_t)n23 * iz] = 1.0;
+    }
+}
+
+__device__ __forceinline__ double tab_at(const Tab<double>& t, int row, int col) {
+    const int j = col - t.start[row];
+    return (j >= 0 && j < t.width[row]) ? t.c[(size_t)row * kTabW + j] : 0.0;
+}
+
+// b = H P (D+SAT) chi(s) on the merged grid, from slip fields on the face grid.
+// chi = -+s/2 across the fault (src/FaultResponse.jl:176-185); the x1 structure of
+// every term reduces to the jump coefficients Jt (tangential components, from s
+// itself) and Jn (normal component, from the strike/dip derivatives of s).
+// Also initialises the Krylov vectors: x = 0, xc = 0, r = p = b.
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_rhs(RhsView rv, int NN, const double* __restrict__ fields, int nslots,
+                                                double* __restrict__ b, double* __restrict__ x, T* __restrict__ r,
+                                                T* __restrict__ p, float* __restrict__ xc) {
+    constexpr int NPB = kBlock / K;
+    const int tile = blockIdx.y, g = blockIdx.x;
+    const int lane = threadIdx.x % K, slot = threadIdx.x / K;
+    const int col = tile * K + lane;
+    const int n23 = rv.n23;
+    const double* s2 = fields + (size_t)col * 2 * n23 * n23;
+    const double* s3 = s2 + (size_t)n23 * n23;
+    const size_t off = (size_t)tile * NN * 3 * K;
+    for (int base = g * NPB; base < NN; base += kGroups * NPB) {
+        const int node = base + slot;
+        if (node >= NN) continue;
+        const int m = node % rv.NX, yz = node / rv.NX, y = yz % rv.NY, z = yz / rv.NY;
+        const int iy = y + 1, iz = z + 1;  // full face indices
+        double v[3] = {0.0, 0.0, 0.0};
+        if (col < nslots) {
+            const double H = rv.Hx[m] * rv.Hy[y] * rv.Hy[z];
+            const double jt = rv.Jt[m], jn = rv.Jn[m];
+            if (jt != 0.0) {
+                v[1] = H * jt * s2[iy + (size_t)n23 * iz];
+                v[2] = H * jt * s3[iy + (size_t)n23 * iz];
+            }
+            if (jn != 0.0) {
+                double d = 0.0;
+                const int sy = rv.D1full.start[y], wy = rv.D1full.width[y];
+                for (int t = 0; t < wy; ++t) d += rv.D1full.c[(size_t)y * kTabW + t] * s2[(sy + t) + (size_t)n23 * iz];
+                const int sz = rv.D1full.start[z], wz = rv.D1full.width[z];
+                for (int t = 0; t < wz; ++t) d += rv.D1full.c[(size_t)z * kTabW + t] * s3[iy + (size_t)n23 * (sz + t)];
+                v[0] = H * jn * d;
+            }
+        }
+        const size_t e = off + (size_t)node * 3 * K + lane;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            b[e + c * K] = v[c];
+            x[e + c * K] = 0.0;
+            r[e + c * K] = (T)v[c];
+            p[e + c * K] = (T)v[c];
+            if (xc) xc[e + c * K] = 0.0f;
+        }
+    }
+}
+
+// Initial per-column state from ||b|| (weighted): Krylov's setup before its loop.
+template <int K>
+__global__ void k_init_state(ColState* __restrict__ cs, const double* __restrict__ partial,
+                             const int64_t* __restrict__ colid, double rtol, double atol, int64_t itmax,
+                             int* __restrict__ tile_on) {
+    const int tile = blockIdx.x, lane = threadIdx.x;
+    ColState& st = cs[(size_t)tile * K + lane];
+    const double s = sum_partials<K>(partial, tile, lane);
+    st.col = colid[(size_t)tile * K + lane];
+    st.rr = s;
+    st.pnorm2 = s;
+    st.bnorm = sqrt(s);
+    st.eps = atol + rtol * st.bnorm;
+    st.alpha = st.beta = 0.0;
+    st.iters = 0;
+    st.since_rel = 0;
+    st.rel_flag = 0;
+    st.itmax = itmax;
+    if (st.col < 0) st.status = kIdle;
+    else if (s == 0.0 || st.bnorm <= st.eps) st.status = kSolved;  // x = 0 is the answer
+    else st.status = kActive;
+    const int on = __syncthreads_or(st.status == kActive);
+    if (lane == 0) tile_on[tile] = on;
+}
+
+// Weighted partial dot of one vector with itself (used for ||b||).
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_selfdot(int NN, int NX, int mface, const T* __restrict__ v,
+                                                    double* __restrict__ partial) {
+    constexpr int NPB = kBlock / K;
+    const int tile = blockIdx.y, g = blockIdx.x;
+    const int lane = threadIdx.x % K, slot = threadIdx.x / K;
+    const size_t off = (size_t)tile * NN * 3 * K;
+    double acc = 0.0;
+    for (int base = g * NPB; base < NN; base += kGroups * NPB) {
+        const int node = base + slot;
+        if (node >= NN) continue;
+        const size_t e = off + (size_t)node * 3 * K + lane;
+        double d = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) d += (double)v[e + c * K] * (double)v[e + c * K];
+        acc += ((node % NX == mface) ? 2.0 : 1.0) * d;
+    }
+    block_partial<K>(acc, partial, tile, g);
+}
+
+// Traction change on Omega_f from the merged solution u and the slip fields:
+// dtau_c = 1/2 (T_c U- + T_c U+), U = P u + chi     (src/FaultResponse.jl:201-209)
+// Writes K[row, col] with row = (c-1)*nf + a + n2f*b (column-major, leading dim ldK).
+template <int K>
+__global__ void k_extract(ExtractView ev, int NN, const double* __restrict__ u, const double* __restrict__ fields,
+                          const int64_t* __restrict__ colid, double* __restrict__ Kout, int64_t ldK,
+                          int64_t col_offset) {
+    const int tile = blockIdx.y;
+    const int lane = threadIdx.x % K;
+    const int k = blockIdx.x * (blockDim.x / K) + threadIdx.x / K;  // Omega_f node
+    if (k >= ev.nf) return;
+    const int64_t col = colid[(size_t)tile * K + lane];
+    if (col < 0) return;
+    const int n23 = ev.n23, NX = ev.NX, NY = ev.NY;
+    const int iy = ev.i2lo + k % ev.n2f, iz = ev.i2lo + k / ev.n2f;
+    const bool yin = iy >= 1 && iy <= n23 - 2, zin = iz >= 1 && iz <= n23 - 2;
+    const double* f = fields + ((size_t)tile * K + lane) * 2 * n23 * n23;
+    const double* ut = u + (size_t)tile * NN * 3 * K;
+    auto ld = [&](int m, int y, int z, int c) { return ut[((size_t)(m + NX * (y + NY * z)) * 3 + c) * K + lane]; };
+    double out[2];
+    for (int d = 0; d < 2; ++d) {
+        const int c = d + 1;
+        double v = ev.Jext * f[(size_t)d * n23 * n23 + iy + (size_t)n23 * iz];
+        if (yin && zin)
+            for (int m = 0; m < NX; ++m) v += ev.Dn[m] * ld(m, iy - 1, iz - 1, c);
+        double tg = 0.0;
+        const int row = d == 0 ? iy : iz;
+        const int s = ev.D1ext.start[row], w = ev.D1ext.width[row];
+        if (d == 0 ? zin : yin)
+            for (int t = 0; t < w; ++t) {
+                const double cc = ev.D1ext.c[(size_t)row * kTabW + t];
+                tg += cc * (d == 0 ? ld(ev.mface, s + t, iz - 1, 0) : ld(ev.mface, iy - 1, s + t, 0));
+            }
+        out[d] = v + ev.mu * tg;
+    }
+    double* Kc = Kout + (size_t)(col - col_offset) * ldK;
+    Kc[k] = out[0];
+    Kc[ev.nf + k] = out[1];
+}
+
+// Plain single-vector kernels (one column, K = 1) reuse the batched ones.
+
+}  // namespace eqd

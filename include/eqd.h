@@ -1,0 +1,154 @@
+/* libeqd -- MI355X (gfx950) split-node fault-stiffness engine and BP8 RHS, C ABI.
+ *
+ * The library replaces the hot path of EarthquakeDiffinitive.jl: the split-node CG
+ * solver (ElasticitySplitNode.CGSolver) and the dense fault stiffness build
+ * (FaultResponse.fault_stiffness, :exact and :toeplitz), plus the BP8 right-hand
+ * side. All signatures use plain pointers and sizes so a Julia host can `ccall`
+ * them; host arrays are Julia's column-major Float64 layout (pointer(K) can be
+ * passed directly).
+ *
+ * Conventions
+ *   - Every function returns an eqd_status; 0 is success. On failure
+ *     eqd_last_error() holds a message (thread-local).
+ *   - Full-space DOF vectors have the reference's layout: [u-(3N); u+(3N)],
+ *     component-major, x1 fastest (src/ElasticitySplitNode.jl:36-42).
+ *   - Slip/traction vectors are indexed over Omega_f nodes, x2 fastest
+ *     (src/FaultResponse.jl:112-126). K is 2nf x 2nf, column-major, rows
+ *     [dtau2; dtau3], columns [s2; s3] (src/FaultResponse.jl:214-232).
+ *   - A solver handle is single-threaded (like Krylov's CgWorkspace); distinct
+ *     handles may be used from different threads. Handles own their device memory.
+ */
+#ifndef EQD_H
+#define EQD_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int eqd_status;
+enum {
+    EQD_OK = 0,
+    EQD_ERR_ARG = 1,        /* invalid argument (ErrorException in the reference) */
+    EQD_ERR_GRID = 2,       /* grid/patch legality (src/FaultResponse.jl:118-124, src/BP8.jl:189-204) */
+    EQD_ERR_DEVICE = 3,     /* HIP runtime failure, or no usable gfx950 device */
+    EQD_ERR_NOMEM = 4,      /* device allocation failed */
+    EQD_ERR_RANGE = 5,      /* rhs not in range(P) */
+    EQD_ERR_NEWTON = 6      /* rate-state Newton solve did not converge (src/RateStateFriction.jl:93-94) */
+};
+
+typedef struct eqd_op eqd_op;
+typedef struct eqd_solver eqd_solver;
+
+/* Two SBP blocks [-L_normal,0]x[-L_fault,L_fault]^2 and [0,L_normal]x[...]^2 with
+ * n1 x n23 x n23 nodes each (src/BP8.jl:196-206); Omega_f = (-l_f, l_f)^2. */
+typedef struct {
+    int n1, n23;
+    double L_normal, L_fault;
+    double lambda, mu;
+    double l_f;
+} eqd_grid_desc;
+
+typedef struct {
+    int64_t nf, n2f, n3f;   /* Omega_f nodes and its two extents */
+    int64_t ntot;           /* full-space DOFs, 6*n1*n23^2 */
+    int64_t nreduced;       /* unknowns the device stores per column (range(P), interior) */
+    double h1, h2;
+} eqd_op_info;
+
+enum { EQD_PREC_FP64 = 0, EQD_PREC_MIXED = 1 };
+enum { EQD_PRECOND_NONE = 0, EQD_PRECOND_JACOBI = 1 };
+enum { EQD_ENGINE_BATCHED = 0, EQD_ENGINE_PLAIN = 1 };
+
+typedef struct {
+    double rtol;              /* default 1e-10 (src/ElasticitySplitNode.jl:365) */
+    double atol;              /* default 0 */
+    int64_t itmax;            /* 0 = 2*ntot, Krylov's default */
+    int precond;              /* EQD_PRECOND_NONE only; JACOBI is rejected (measured 0.92x in the reference) */
+    int precision;            /* EQD_PREC_FP64 or EQD_PREC_MIXED (FP32 CG inside FP64 reliable updates) */
+    int engine;               /* EQD_ENGINE_BATCHED (device-side alpha/beta) or _PLAIN (host alpha/beta, one column) */
+    int tile_k;               /* columns interleaved per tile: 1, 8, 16, 32 or 64 (0 = autotuned) */
+    int max_cols_in_flight;   /* columns resident at once (0 = all, bounded by memory) */
+    int reliable_interval;    /* mixed precision: iterations between FP64 residual refreshes (0 = 16) */
+    int graph_iters;          /* iterations captured per HIP graph replay (0 = 8) */
+    int timing;               /* 1: no graphs, HIP events around every kernel (measurement runs) */
+} eqd_cg_opts;
+
+typedef struct {
+    int64_t solves, iterations, max_iterations, unconverged;
+    double mean_iterations;
+    int precond;
+} eqd_report;
+
+void eqd_cg_opts_default(eqd_cg_opts* o);
+
+/* Operator: assembles the 1D tables of -HP(D+SAT)P, HP(D+SAT) and the traction map.
+ * Replaces split_node_system + FaultElasticity setup (src/ElasticitySplitNode.jl:106-214,
+ * src/FaultResponse.jl:96-146). Errors: n1 or n23 < 9, no grid node on +-l_f. */
+eqd_status eqd_operator_create(const eqd_grid_desc* g, eqd_op** out);
+eqd_status eqd_operator_destroy(eqd_op* op);
+eqd_status eqd_operator_info(const eqd_op* op, eqd_op_info* info);
+/* Omega_f coordinates (x2f[n2f], x3f[n3f]); fault_grid_axes (src/FaultResponse.jl:161). */
+eqd_status eqd_fault_grid_axes(const eqd_op* op, double* x2f, double* x3f);
+
+/* y = A x with A = -HP(D+SAT)P, full-space FP64 vectors (A's product as used by cg!). */
+eqd_status eqd_apply_A(eqd_op* op, const double* x, double* y);
+/* rhs = HP(D+SAT) chi(s) for slip on Omega_f (src/FaultResponse.jl:202-203), full space. */
+eqd_status eqd_build_rhs(eqd_op* op, const double* s2, const double* s3, double* rhs);
+
+/* CGSolver(A; rtol, atol, itmax, precond) (src/ElasticitySplitNode.jl:365). */
+eqd_status eqd_solver_create(eqd_op* op, const eqd_cg_opts* opts, eqd_solver** out);
+eqd_status eqd_solver_destroy(eqd_solver* s);
+/* duplicate(solver): shares the operator, fresh workspace and counters (:408). */
+eqd_status eqd_solver_duplicate(const eqd_solver* s, eqd_solver** out);
+/* merge_stats!(into, from) (:426). */
+eqd_status eqd_solver_merge_stats(eqd_solver* into, const eqd_solver* from);
+/* solver_report(solver) (:442). */
+eqd_status eqd_solver_report(const eqd_solver* s, eqd_report* r);
+
+/* split_node_solve(solver, rhs) -> u (:378). rhs must lie in range(P), as every rhs
+ * HP(D+SAT)chi does; otherwise EQD_ERR_RANGE. u is the full-space iterate (P u = u). */
+eqd_status eqd_split_node_solve(eqd_solver* s, const double* rhs, double* u);
+
+/* shear_traction(fe, s2, s3) -> (dtau2, dtau3) (src/FaultResponse.jl:194-211). */
+eqd_status eqd_shear_traction(eqd_solver* s, const double* s2, const double* s3, double* dtau2, double* dtau3);
+
+enum { EQD_STIFFNESS_EXACT = 0, EQD_STIFFNESS_TOEPLITZ = 1 };
+/* fault_stiffness(fe) / fault_stiffness_toeplitz(fe) (src/FaultResponse.jl:229-287, 339-393).
+ * EXACT: cols == NULL builds all 2nf columns into K (ldK >= 2nf, 2nf columns);
+ *        otherwise only columns cols[0..ncols) (0-based) are built, into K[:, 0..ncols).
+ * TOEPLITZ: 10 solves, full K (cols must be NULL). */
+eqd_status eqd_fault_stiffness(eqd_solver* s, int mode, const int* cols, int ncols, double* K, int64_t ldK);
+/* Same, with K left in device memory (caller-owned device pointer). */
+eqd_status eqd_fault_stiffness_device(eqd_solver* s, int mode, const int* cols, int ncols, double* K_dev,
+                                      int64_t ldK);
+
+/* Timing hooks for measurement: device time (ms) of the last stiffness build's
+ * iteration phase, and the number of stencil launches and tile-iterations in it. */
+eqd_status eqd_solver_last_timing(const eqd_solver* s, double* ms, int64_t* stencil_launches,
+                                  int64_t* tile_iterations);
+
+/* Measurement hook, no counterpart in the reference: per-kernel device time of the last
+ * stiffness build run with opts.timing = 1. kernel: 0 stencil q = A p, 1 x/r update,
+ * 2 p update, 3 per-column finalize, 4 reliable-update accumulate, 5 reliable-update
+ * FP64 residual. Returns summed ms, launch count and the sum over launches of the
+ * number of tiles with an active column (each tile holds tile_k columns). */
+eqd_status eqd_solver_kernel_timing(const eqd_solver* s, int kernel, double* total_ms, int64_t* launches,
+                                    int64_t* active_tiles);
+
+/* Diagnostics, no counterpart in the reference: the host-built 1D tables behind the
+ * kernels, available without a device (for inspection and CPU-side tests of the
+ * table construction). which: 0 AXs, 1 AXo, 2 MX, 3 Ts, 4 To, 5 D1, 6 D1s,
+ * 7 D1full, 8 D1ext (banded: start[nrows], width[nrows], coef[nrows*16]);
+ * 100 Hx, 101 Hy, 102 Jt, 103 Jn, 104 Dn (vectors into coef), 105 Jext (scalar).
+ * Any output pointer may be NULL. Returns the row count / vector length, or -1. */
+int64_t eqd_tables_export(const eqd_grid_desc* g, int which, int* start, int* width, double* coef);
+
+/* Version and last error. */
+const char* eqd_version(void);
+size_t eqd_last_error(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,179 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars: operators (A, HP(D+SAT)chi, traction map) to 1e-13 relative; solutions and K
+within 1e-8 relative (Frobenius) of the oracle's FP64 CG, the north-star bound for
+K; zero slip -> exactly zero traction (test/fault_response_test.jl:74-78); batched
+builds bitwise independent of how columns are batched (the reference's threaded ==
+serial property, test/fault_response_test.jl:100-112)."""
+import numpy as np
+import pytest
+
+import eqd
+from eqd import _lib
+from oracle.oracle import OP_A, OP_HPDSAT, OP_P, Geometry, OracleSystem
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "cube11": (11, 11, 1.0, 1.0, 2.0, 1.0, 0.4),
+    "skew": (9, 13, 1.0, 1.2, 2.0, 1.0, 0.6),
+    "ring": (9, 17, 400.0, 800.0, 32.04e9, 32.04e9, 800.0),   # l_f = L_fault
+}
+
+
+def pair(name, **kw):
+    n1, n23, Ln, Lf, lam, mu, lf = CASES[name]
+    print(f"[case {name}] {kw}", flush=True)
+    fe = eqd.FaultElasticity(eqd.GridPair(n1, n23, Ln, Lf), lam, mu, l_f=lf, **kw)
+    return fe, OracleSystem(Geometry(n1, n23, Ln, Lf, lam, mu, lf))
+
+
+_ORACLE_K = {}
+
+
+def oracle_K(name):
+    if name not in _ORACLE_K:
+        n1, n23, Ln, Lf, lam, mu, lf = CASES[name]
+        _ORACLE_K[name] = OracleSystem(Geometry(n1, n23, Ln, Lf, lam, mu, lf)).fault_stiffness(nthreads=8)
+    return _ORACLE_K[name]
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_apply_A_matches_assembled(name):
+    fe, o = pair(name)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(o.ntot)
+    y = fe.apply_A(x)
+    yo = o.apply(OP_A, x)
+    assert np.abs(y - yo).max() <= 1e-12 * np.abs(yo).max()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_rhs_matches_assembled(name):
+    fe, o = pair(name)
+    rng = np.random.default_rng(4)
+    s2, s3 = rng.standard_normal(o.nf), rng.standard_normal(o.nf)
+    b = fe.build_rhs(s2, s3)
+    bo = o.apply(OP_HPDSAT, o.build_chi(s2, s3))
+    assert np.abs(b - bo).max() <= 1e-12 * np.abs(bo).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_split_node_solve(precision):
+    fe, o = pair("cube11", precision=precision)
+    nf = o.nf
+    s2 = np.array([0.01 * np.sin(3 * (k + 1)) for k in range(nf)])
+    s3 = np.array([0.02 * np.cos(2 * (k + 1)) for k in range(nf)])
+    b = o.apply(OP_HPDSAT, o.build_chi(s2, s3))
+    u = eqd.split_node_solve(fe.rs, b)
+    uo, it, ok = o.cg(b)
+    assert rel(u, uo) < 1e-8
+    assert np.array_equal(u, o.apply(OP_P, u)) or rel(o.apply(OP_P, u), u) < 1e-15
+    assert np.linalg.norm(b - o.apply(OP_A, u)) / np.linalg.norm(b) < 1e-8
+    rep = eqd.solver_report(fe.rs)
+    assert rep["solves"] == 1 and rep["unconverged"] == 0
+    if precision == "fp64":
+        assert abs(rep["iterations"] - it) <= 2
+
+
+def test_split_node_solve_rejects_rhs_outside_range_P():
+    fe, o = pair("cube11")
+    with pytest.raises(eqd.EqdError) as e:
+        eqd.split_node_solve(fe.rs, np.ones(o.ntot))
+    assert e.value.code == _lib.EQD_ERR_RANGE
+
+
+def test_shear_traction_and_zero_slip():
+    fe, o = pair("cube11")
+    nf = o.nf
+    s2 = np.array([0.01 * np.sin(3 * (k + 1)) for k in range(nf)])
+    s3 = np.array([0.02 * np.cos(2 * (k + 1)) for k in range(nf)])
+    d2, d3 = eqd.shear_traction(fe, s2, s3)
+    o2, o3 = o.shear_traction(s2, s3)
+    assert rel(d2, o2) < 1e-8 and rel(d3, o3) < 1e-8
+    z2, z3 = eqd.shear_traction(fe, np.zeros(nf), np.zeros(nf))
+    assert not z2.any() and not z3.any()
+    p2, _ = eqd.shear_traction(fe, np.ones(nf), np.zeros(nf))
+    assert p2.max() < 0
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("precision,tile_k", [("fp64", 64), ("fp64", 16), ("mixed", 64), ("mixed", 8)])
+def test_fault_stiffness_exact(name, precision, tile_k):
+    fe, o = pair(name, precision=precision, tile_k=tile_k)
+    K = eqd.fault_stiffness(fe)
+    Ko = oracle_K(name)
+    assert rel(K, Ko) < 1e-8
+    assert np.all(np.diag(K)[: o.nf] < 0) or name == "ring"
+    rep = eqd.solver_report(fe.rs)
+    assert rep["solves"] == 2 * o.nf and rep["unconverged"] == 0
+
+
+def test_plain_engine_matches_oracle():
+    fe, o = pair("cube11", engine="plain")
+    cols = [0, 5, o.nf + 3]
+    K = eqd.fault_stiffness(fe, cols=cols)
+    assert rel(K, oracle_K("cube11")[:, cols]) < 1e-8
+
+
+def test_stiffness_matches_forward_solve():
+    fe, o = pair("cube11", precision="mixed")
+    K = eqd.fault_stiffness(fe)
+    nf = o.nf
+    s2 = np.array([0.01 * np.sin(3 * (k + 1)) for k in range(nf)])
+    s3 = np.array([0.02 * np.cos(2 * (k + 1)) for k in range(nf)])
+    d2, d3 = eqd.shear_traction(fe, s2, s3)
+    st = K @ np.concatenate([s2, s3])
+    assert rel(st[:nf], d2) < 1e-8 and rel(st[nf:], d3) < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batching_is_bitwise_invariant(precision):
+    """Columns computed in one batch, in small batches, or as a subset are identical bits."""
+    fe, o = pair("cube11", precision=precision, tile_k=16)
+    K_all = eqd.fault_stiffness(fe)
+    it_all = eqd.solver_report(fe.rs)["iterations"]
+    fe2, _ = pair("cube11", precision=precision, tile_k=16, max_cols_in_flight=32)
+    K_small = eqd.fault_stiffness(fe2)
+    assert np.array_equal(K_all, K_small)
+    assert eqd.solver_report(fe2.rs)["iterations"] == it_all
+    cols = [7, 3, o.nf + 11, 0]
+    fe3, _ = pair("cube11", precision=precision, tile_k=16)
+    assert np.array_equal(eqd.fault_stiffness(fe3, cols=cols), K_all[:, cols])
+
+
+def test_toeplitz_matches_oracle_toeplitz():
+    fe, o = pair("cube11", precision="mixed")
+    Kt = eqd.fault_stiffness_toeplitz(fe)
+    Kto = o.fault_stiffness_toeplitz()
+    assert rel(Kt, Kto) < 1e-8
+    assert eqd.solver_report(fe.rs)["solves"] == 10
+    K = oracle_K("cube11")
+    assert rel(Kt, K) < 0.05
+
+
+def test_unconverged_is_counted_not_thrown():
+    fe, o = pair("cube11", itmax=1)
+    K = eqd.fault_stiffness(fe, cols=[0, 1])
+    assert eqd.solver_report(fe.rs)["unconverged"] == 2
+    assert np.isfinite(K).all()
+
+
+def test_duplicate_and_merge_stats():
+    fe, o = pair("cube11")
+    d = eqd.duplicate(fe.rs)
+    eqd.fault_stiffness(fe, cols=[0, 1], solver=d)
+    assert eqd.solver_report(fe.rs)["solves"] == 0
+    eqd.merge_stats(fe.rs, d)
+    assert eqd.solver_report(fe.rs)["solves"] == 2
+
+
+def test_jacobi_rejected_unknown_precond_errors():
+    with pytest.raises(eqd.EqdError):
+        pair("cube11", precond="jacobi")
+    with pytest.raises(eqd.EqdError):
+        pair("cube11", precond="ilu")
