@@ -142,6 +142,22 @@ struct eqd_op {
         return OpView<V>{tb.NX, tb.NY, tb.mface, tab<V>(tb.AXs), tab<V>(tb.AXo), tab<V>(tb.MX), tab<V>(tb.Ts),
                          tab<V>(tb.To), tab<V>(tb.D1), tab<V>(tb.D1s), up(hx), up(hy)};
     }
+    template <typename V>
+    PencilView<V> pview() {
+        constexpr int S = pencil_s<V>();
+        const int nseg = (tb.NX + S - 1) / S;
+        auto band = [&](const HostTab& t, int R) {  // x1 bands padded with zero rows to nseg*S
+            std::vector<double> b = build_band(t, R);
+            if (t.nrows == tb.NX) b.resize((size_t)nseg * S * (2 * R + 1), 0.0);
+            return up(std::vector<V>(b.begin(), b.end()));
+        };
+        std::vector<V> hx(tb.Hx.begin(), tb.Hx.end()), hy(tb.Hy.begin(), tb.Hy.end());
+        return PencilView<V>{tb.NX, tb.NY, tb.mface, nseg, band(tb.AXs, kRxS),
+                             band(tb.AXo, kRxO), band(tb.MX, kRxO), band(tb.Ts, kRyS), band(tb.To, kRyO),
+                             band(tb.D1, kRyO), band(tb.D1s, kRyO), up(hx), up(hy)};
+    }
+    PencilView<double> p64{};
+    PencilView<float> p32{};
     int64_t lin(int i1, int i2, int i3) const { return i1 + (int64_t)g.n1 * (i2 + (int64_t)g.n23 * i3); }
 };
 
@@ -351,7 +367,16 @@ struct Engine {
             launch();
             if (kt) kt->end(id, st);
         };
-        timed(0, [&] { hipLaunchKernelGGL((k_apply_dot<T, K>), grid, dim3(kBlock), 0, st, v, op.NN, p, q, on, part); });
+        timed(0, [&] {
+            if constexpr (K == 64) {
+                PencilView<T> pv;
+                if constexpr (std::is_same<T, float>::value) pv = op.p32; else pv = op.p64;
+                hipLaunchKernelGGL((k_pencil<T, T, 0>), grid, dim3(kBlock), pencil_lds_bytes(pv), st, pv, op.NN, p, q,
+                                   (const T*)nullptr, (T*)nullptr, cs, on, part);
+            } else {
+                hipLaunchKernelGGL((k_apply_dot<T, K>), grid, dim3(kBlock), 0, st, v, op.NN, p, q, on, part);
+            }
+        });
         timed(3, [&] {
             hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, 0, (int)mixed, R, cs, part, on, rel,
                                nullptr);
@@ -375,8 +400,14 @@ struct Engine {
                                    cs, rel, 0);
             });
             timed(5, [&] {
-                hipLaunchKernelGGL((k_residual<T, K>), grid, dim3(kBlock), 0, st, op.v64, op.NN, w.x.as<double>(),
-                                   w.b.as<double>(), r, cs, rel, part);
+                if constexpr (K == 64)
+                    hipLaunchKernelGGL((k_pencil<double, T, 1>), grid, dim3(kBlock), pencil_lds_bytes(op.p64), st,
+                                       op.p64, op.NN,
+                                       (const double*)w.x.as<double>(), (double*)nullptr,
+                                       (const double*)w.b.as<double>(), r, cs, rel, part);
+                else
+                    hipLaunchKernelGGL((k_residual<T, K>), grid, dim3(kBlock), 0, st, op.v64, op.NN, w.x.as<double>(),
+                                       w.b.as<double>(), r, cs, rel, part);
             });
             timed(3, [&] {
                 hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, 2, 1, R, cs, part, on, rel,
@@ -496,7 +527,9 @@ struct Engine {
         dim3 grid(kGroups, 1);
         std::vector<double> part(kGroups);
         ColState h;
-        HIPCHK(hipMemcpy(&h, w.cs.p, sizeof(ColState), hipMemcpyDeviceToHost));
+        // the state was initialised on the solver's (non-blocking) stream
+        HIPCHK(hipMemcpyAsync(&h, w.cs.p, sizeof(ColState), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
@@ -634,7 +667,10 @@ void run_columns(eqd_solver* s, const std::vector<int64_t>& colid, const std::ve
         per = (int)std::max<size_t>(1, std::min<size_t>((size_t)per, cap));
     }
     if (s->o.engine == EQD_ENGINE_PLAIN) per = 1;
-    const int ntiles = (std::min(per, n) + K - 1) / K;
+    int ntiles = (std::min(per, n) + K - 1) / K;
+    // a multiple of 8 tiles lets the kernels deal tiles to the 8 XCDs (tile_group);
+    // the extra tiles hold idle columns and are skipped
+    if (ntiles > 1) ntiles = (ntiles + 7) / 8 * 8;
     ensure_ws(s, K, ntiles, prec);
     Workspace& w = *s->ws;
     const int slots = ntiles * K;
@@ -856,6 +892,8 @@ eqd_status eqd_operator_create(const eqd_grid_desc* g, eqd_op** out) {
         op->nf = cnt * cnt;
         op->v64 = op->view<double>();
         op->v32 = op->view<float>();
+        op->p64 = op->pview<double>();
+        op->p32 = op->pview<float>();
         op->rv = RhsView{op->tb.NX, op->tb.NY, g->n23, lo, cnt, op->nf, op->up(op->tb.Jt), op->up(op->tb.Jn),
                          op->v64.Hx, op->v64.Hy, op->tab<double>(op->tb.D1full)};
         op->ev = ExtractView{op->tb.NX, op->tb.NY, op->tb.mface, g->n23, lo, cnt, op->nf, g->mu, op->tb.Jext,

@@ -51,6 +51,30 @@ struct HostTables {
 
 HostTables build_tables(int n1, int n23, double L_normal, double L_fault, double lambda, double mu);
 
+// Fixed-reach band form of a table: band[r*(2R+1) + R + d] = coefficient of column r + d.
+// The reaches below hold for every grid (they are set by the SBP-4 closures restricted
+// to interior columns); build_band() throws if a table exceeds its reach.
+constexpr int kRxS = 5;  // AXs: wide D1*D1 along x1 plus the SAT normal closure
+constexpr int kRxO = 3;  // AXo, MX
+constexpr int kRyS = 4;  // Ts
+constexpr int kRyO = 2;  // To, D1, D1s
+std::vector<double> build_band(const HostTab& t, int R);
+
+// Device view of the band tables used by the pencil kernel.
+template <typename T>
+struct PencilView {
+    int NX, NY, mface, nseg;
+    const T* AXs;  // NX x (2*kRxS+1)
+    const T* AXo;  // NX x (2*kRxO+1)
+    const T* MX;   // NX x (2*kRxO+1)
+    const T* Ts;   // NY x (2*kRyS+1)
+    const T* To;   // NY x (2*kRyO+1)
+    const T* D1;
+    const T* D1s;
+    const T* Hx;
+    const T* Hy;
+};
+
 // Device view of one table (typed coefficients).
 template <typename T>
 struct Tab {

@@ -33,6 +33,22 @@ struct ColState {
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// (tile, group) of this block for the node-parallel grid (kGroups x ntiles). Blocks are
+// dispatched round-robin over the 8 XCDs in linear order; when ntiles is a multiple of
+// 8 the tiles are dealt to XCDs the same way, so each XCD's L2 (and its share of the
+// Infinity Cache) only ever holds its own tiles' vectors.
+__device__ __forceinline__ void tile_group(int& tile, int& g) {
+    if ((gridDim.y & 7) == 0 && gridDim.x == kGroups) {
+        const int b = blockIdx.x + gridDim.x * blockIdx.y;
+        const int xcd = b & 7, q = b >> 3;
+        tile = ((q / kGroups) << 3) + xcd;
+        g = q % kGroups;
+    } else {
+        tile = blockIdx.y;
+        g = blockIdx.x;
+    }
+}
+
 // q = -H (reduced split-node operator) p for the node (m, y, z) of one column.
 template <typename T, int K>
 __device__ __forceinline__ void stencil_node(const OpView<T>& op, const T* __restrict__ in, int m, int y, int z,
@@ -147,7 +163,8 @@ __global__ __launch_bounds__(kBlock) void k_apply_dot(OpView<T> op, int NN, cons
                                                       T* __restrict__ q, const int* __restrict__ tile_on,
                                                       double* __restrict__ partial) {
     constexpr int NPB = kBlock / K;
-    const int tile = blockIdx.y, g = blockIdx.x;
+    int tile, g;
+    tile_group(tile, g);
     const int lane = threadIdx.x % K, slot = threadIdx.x / K;
     double acc = 0.0;
     if (tile_on[tile]) {
@@ -182,7 +199,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(OpView<double> op, int NN, 
                                                      const ColState* __restrict__ cs, const int* __restrict__ tile_on,
                                                      double* __restrict__ partial) {
     constexpr int NPB = kBlock / K;
-    const int tile = blockIdx.y, g = blockIdx.x;
+    int tile, g;
+    tile_group(tile, g);
     const int lane = threadIdx.x % K, slot = threadIdx.x / K;
     const bool mine = cs[(size_t)tile * K + lane].rel_flag != 0;
     double acc = 0.0;
@@ -217,7 +235,8 @@ __global__ __launch_bounds__(kBlock) void k_update(int NN, int NX, int mface, X*
                                                    const T* __restrict__ q, const ColState* __restrict__ cs,
                                                    const int* __restrict__ tile_on, double* __restrict__ partial) {
     constexpr int NPB = kBlock / K;
-    const int tile = blockIdx.y, g = blockIdx.x;
+    int tile, g;
+    tile_group(tile, g);
     const int lane = threadIdx.x % K, slot = threadIdx.x / K;
     const ColState& st = cs[(size_t)tile * K + lane];
     double acc = 0.0;
@@ -250,7 +269,8 @@ template <typename T, int K>
 __global__ __launch_bounds__(kBlock) void k_pupdate(int NN, T* __restrict__ p, const T* __restrict__ r,
                                                     const ColState* __restrict__ cs, const int* __restrict__ tile_on) {
     constexpr int NPB = kBlock / K;
-    const int tile = blockIdx.y, g = blockIdx.x;
+    int tile, g;
+    tile_group(tile, g);
     const int lane = threadIdx.x % K, slot = threadIdx.x / K;
     const ColState& st = cs[(size_t)tile * K + lane];
     if (!tile_on[tile] || st.status != kActive) return;
@@ -271,7 +291,8 @@ __global__ __launch_bounds__(kBlock) void k_xaccum(int NN, double* __restrict__ 
                                                    const ColState* __restrict__ cs, const int* __restrict__ tile_on,
                                                    int all) {
     constexpr int NPB = kBlock / K;
-    const int tile = blockIdx.y, g = blockIdx.x;
+    int tile, g;
+    tile_group(tile, g);
     const int lane = threadIdx.x % K, slot = threadIdx.x / K;
     if (!all && (!tile_on[tile] || !cs[(size_t)tile * K + lane].rel_flag)) return;
     const size_t off = (size_t)tile * NN * 3 * K;
@@ -387,7 +408,8 @@ __global__ __launch_bounds__(kBlock) void k_rhs(RhsView rv, int NN, const double
                                                 double* __restrict__ b, double* __restrict__ x, T* __restrict__ r,
                                                 T* __restrict__ p, float* __restrict__ xc) {
     constexpr int NPB = kBlock / K;
-    const int tile = blockIdx.y, g = blockIdx.x;
+    int tile, g;
+    tile_group(tile, g);
     const int lane = threadIdx.x % K, slot = threadIdx.x / K;
     const int col = tile * K + lane;
     const int n23 = rv.n23;
@@ -458,7 +480,8 @@ template <typename T, int K>
 __global__ __launch_bounds__(kBlock) void k_selfdot(int NN, int NX, int mface, const T* __restrict__ v,
                                                     double* __restrict__ partial) {
     constexpr int NPB = kBlock / K;
-    const int tile = blockIdx.y, g = blockIdx.x;
+    int tile, g;
+    tile_group(tile, g);
     const int lane = threadIdx.x % K, slot = threadIdx.x / K;
     const size_t off = (size_t)tile * NN * 3 * K;
     double acc = 0.0;
@@ -513,6 +536,12 @@ __global__ void k_extract(ExtractView ev, int NN, const double* __restrict__ u, 
     Kc[k] = out[0];
     Kc[ev.nf + k] = out[1];
 }
+
+}  // namespace eqd
+
+#include "pencil.h"
+
+namespace eqd {
 
 // Plain single-vector kernels (one column, K = 1) reuse the batched ones.
 

@@ -1,0 +1,312 @@
+// Pencil stencil for tiles of 64 columns. Included by kernels.h.
+//
+// One wavefront = one x1 pencil segment of S nodes at fixed (x2, x3); lanes = the 64
+// right-hand sides of a tile. Every coefficient is wave-uniform: the x2/x3 bands come
+// through the scalar cache, the x1 bands from LDS (broadcast reads). The x1 operators
+// are applied from a register window; the x2/x3 operators are uniform-coefficient line
+// updates, so one loaded line serves every output component that needs it. The mixed
+// x1-x2 and x1-x3 terms are sum-factorised: the x2 (x3) derivative is accumulated over
+// the x1 window first and the x1 band is applied once.
+//
+// The tile's vector is read through one buffer resource, so neighbour-line loads need
+// no per-load address registers. Register discipline: every component index is a template parameter (no dynamic
+// indexing of register arrays), the loops over neighbour lines are not unrolled (one
+// line's loads in flight per wave; occupancy supplies the rest), and scheduling fences
+// keep one output row's coefficients from being hoisted into the next.
+//
+// The inner loops are branch-free: x1 positions outside [0, NX) read a clamped (valid)
+// address and meet a zero band coefficient (the tables only couple interior columns),
+// and the x1 bands are zero-padded to nseg*S rows so the unused outputs of the last
+// segment stay in bounds.
+#pragma once
+
+#ifndef EQD_PENCIL_WAVES
+#define EQD_PENCIL_WAVES 4
+#endif
+
+namespace eqd {
+
+// Pencil segment length: 16 nodes in FP32, 8 in FP64 (register budget).
+template <typename T>
+constexpr int pencil_s() { return sizeof(T) == 8 ? 8 : 16; }
+
+#define EQD_FENCE() __builtin_amdgcn_sched_barrier(0)
+
+// Unroll factor of the loops over neighbour lines (lines in flight per wave).
+#ifndef EQD_LINE_UNROLL
+#define EQD_LINE_UNROLL 1
+#endif
+
+// Buffer addressing: one resource per tile vector (base and size in scalar registers),
+// a per-lane byte offset in one VGPR and the wave-uniform element offset in an SGPR,
+// so a neighbour-line load costs no address registers.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t eqd_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// x1 band tables staged in LDS by k_pencil (AXs, AXo, MX for every padded row).
+template <typename T>
+struct XBands {
+    const T* AXs;
+    const T* AXo;
+    const T* MX;
+};
+
+template <typename T, int S>
+struct PencilCtx {
+    const PencilView<T>& op;
+    const XBands<T>& xb;
+    __amdgpu_buffer_rsrc_t rs;
+    int voff;  // lane * sizeof(T)
+    int m0, y, z;
+    static constexpr int C = 3 * 64 * (int)sizeof(T);  // byte stride between consecutive x1 nodes
+    // byte offset of node (0, yy, zz), component c
+    __device__ int line(int yy, int zz, int c) const {
+        return ((op.NX * (yy + op.NY * zz)) * 3 + c) * 64 * (int)sizeof(T);
+    }
+    __device__ int xc(int mm) const { return mm < 0 ? 0 : (mm >= op.NX ? op.NX - 1 : mm); }
+    __device__ T ld(int lineoff, int mm) const { return bload<T>(rs, voff, lineoff + xc(mm) * C); }
+};
+
+// x1 term of component Cc: AXs for component 0, AXo for 1 and 2.
+template <typename T, int S, int Cc>
+__device__ __forceinline__ void pen_x(const PencilCtx<T, S>& P, T (&o)[3][S]) {
+    constexpr int RB = Cc == 0 ? kRxS : kRxO;
+    constexpr int R = kRxS, W = S + 2 * R;
+    EQD_FENCE();
+    const int L = P.line(P.y, P.z, Cc);
+    T w[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) w[j] = P.ld(L, P.m0 - R + j);
+    const T* band = Cc == 0 ? P.xb.AXs : P.xb.AXo;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        // A real (uniform) branch per row: keeps the compiler from hoisting every row's
+        // coefficient reads to the top of the block (which needs hundreds of registers).
+        if (P.m0 + i < P.op.NX) {
+            const T* b = band + (size_t)(P.m0 + i) * (2 * RB + 1);
+            T a = T(0);
+#pragma unroll
+            for (int d = -RB; d <= RB; ++d) a += b[RB + d] * w[i + R + d];
+            o[Cc][i] += a;
+        }
+    }
+}
+
+// Wave-uniform read of a band coefficient through the constant address space, so it is
+// a scalar load (counted on lgkmcnt): a vector load here would share vmcnt with the
+// data loads and every wait for a coefficient would drain all loads in flight.
+template <typename T>
+__device__ __forceinline__ T ucoef(const T* p) {
+    using CP = const __attribute__((address_space(4))) T*;
+    return *(CP)p;
+}
+
+// Neighbour rows outside [0, NY) are clamped to a valid row: their band coefficients
+// are zero (the tables only couple interior rows), so the loads stay branch-free.
+__device__ __forceinline__ int rclamp(int r, int n) { return r < 0 ? 0 : (r >= n ? n - 1 : r); }
+
+// Mixed x1-x2 (AX = 0) or x1-x3 (AX = 1): o[DST] += MX (D1 along AX of component SRC).
+template <typename T, int S, int AX, int SRC, int DST>
+__device__ __forceinline__ void pen_mixed_x(const PencilCtx<T, S>& P, T (&o)[3][S]) {
+    constexpr int R = kRxO, W = S + 2 * R;
+    EQD_FENCE();
+    T a[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) a[j] = T(0);
+    const int row = AX == 0 ? P.y : P.z;
+    const T* bd = P.op.D1 + (size_t)row * (2 * kRyO + 1);
+#pragma unroll EQD_LINE_UNROLL
+    for (int d = -kRyO; d <= kRyO; ++d) {
+        const T cd = ucoef(bd + kRyO + d);
+        const int rr = rclamp(row + d, P.op.NY);
+        const int L = AX == 0 ? P.line(rr, P.z, SRC) : P.line(P.y, rr, SRC);
+        T v[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j) v[j] = P.ld(L, P.m0 - R + j);
+        EQD_FENCE();  // issue the whole line before the first use
+#pragma unroll
+        for (int j = 0; j < W; ++j) a[j] += cd * v[j];
+    }
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        if (P.m0 + i < P.op.NX) {  // per-row branch, as in pen_x
+            const T* b = P.xb.MX + (size_t)(P.m0 + i) * (2 * kRxO + 1);
+            T s = T(0);
+#pragma unroll
+            for (int d = -kRxO; d <= kRxO; ++d) s += b[kRxO + d] * a[i + R + d];
+            o[DST][i] += s;
+        }
+    }
+}
+
+// x2 (AX = 0) or x3 (AX = 1) second-derivative terms: Ts on the component along AX,
+// To on the other two.
+template <typename T, int S, int AX>
+__device__ __forceinline__ void pen_yz(const PencilCtx<T, S>& P, T (&o)[3][S]) {
+    constexpr int SC = AX == 0 ? 1 : 2, O1 = 0, O2 = AX == 0 ? 2 : 1;
+    EQD_FENCE();
+    const int row = AX == 0 ? P.y : P.z;
+    const T* bs = P.op.Ts + (size_t)row * (2 * kRyS + 1);
+    const T* bo = P.op.To + (size_t)row * (2 * kRyO + 1);
+    auto lines = [&](int rr, int c) { return AX == 0 ? P.line(rr, P.z, c) : P.line(P.y, rr, c); };
+#pragma unroll EQD_LINE_UNROLL
+    for (int d = -kRyS; d <= kRyS; ++d) {  // wide same-direction term
+        const T cs = ucoef(bs + kRyS + d);
+        const int L = lines(rclamp(row + d, P.op.NY), SC);
+        T v[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) v[i] = P.ld(L, P.m0 + i);
+        EQD_FENCE();
+#pragma unroll
+        for (int i = 0; i < S; ++i) o[SC][i] += cs * v[i];
+    }
+#pragma unroll EQD_LINE_UNROLL
+    for (int d = -kRyO; d <= kRyO; ++d) {  // narrow term on the other two components
+        const T co = ucoef(bo + kRyO + d);
+        const int rr = rclamp(row + d, P.op.NY);
+        const int La = lines(rr, O1), Lb = lines(rr, O2);
+        T va[S], vb[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            va[i] = P.ld(La, P.m0 + i);
+            vb[i] = P.ld(Lb, P.m0 + i);
+        }
+        EQD_FENCE();
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            o[O1][i] += co * va[i];
+            o[O2][i] += co * vb[i];
+        }
+    }
+}
+
+// Mixed x2-x3: (lam+mu) D1y D1z, coupling components 1 and 2.
+template <typename T, int S>
+__device__ __forceinline__ void pen_mixed_yz(const PencilCtx<T, S>& P, T (&o)[3][S]) {
+    EQD_FENCE();
+    const T* by = P.op.D1s + (size_t)P.y * (2 * kRyO + 1);
+    const T* bz = P.op.D1 + (size_t)P.z * (2 * kRyO + 1);
+#pragma unroll EQD_LINE_UNROLL
+    for (int t = 0; t < (2 * kRyO + 1) * (2 * kRyO + 1); ++t) {
+        const int dy = t / (2 * kRyO + 1) - kRyO, dz = t % (2 * kRyO + 1) - kRyO;
+        const T cc = ucoef(by + kRyO + dy) * ucoef(bz + kRyO + dz);
+        const int yy = rclamp(P.y + dy, P.op.NY), zz = rclamp(P.z + dz, P.op.NY);
+        const int L1 = P.line(yy, zz, 1), L2 = P.line(yy, zz, 2);
+        T v1[S], v2[S];
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            v2[i] = P.ld(L2, P.m0 + i);
+            v1[i] = P.ld(L1, P.m0 + i);
+        }
+        EQD_FENCE();
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            o[1][i] += cc * v2[i];
+            o[2][i] += cc * v1[i];
+        }
+    }
+}
+
+template <typename T, int S>
+__device__ __forceinline__ void pencil_apply(const PencilView<T>& op, const XBands<T>& xb, __amdgpu_buffer_rsrc_t rs,
+                                             int m0, int y, int z, int lane, T (&o)[3][S]) {
+    const PencilCtx<T, S> P{op, xb, rs, lane * (int)sizeof(T), m0, y, z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int i = 0; i < S; ++i) o[c][i] = T(0);
+    pen_x<T, S, 0>(P, o);
+    pen_x<T, S, 1>(P, o);
+    pen_x<T, S, 2>(P, o);
+    pen_mixed_x<T, S, 0, 0, 1>(P, o);  // out1 += (lam+mu) D1x D1y u0
+    pen_mixed_x<T, S, 0, 1, 0>(P, o);  // out0 += (lam+mu) D1x D1y u1
+    pen_mixed_x<T, S, 1, 0, 2>(P, o);  // out2 += (lam+mu) D1x D1z u0
+    pen_mixed_x<T, S, 1, 2, 0>(P, o);  // out0 += (lam+mu) D1x D1z u2
+    pen_yz<T, S, 0>(P, o);
+    pen_yz<T, S, 1>(P, o);
+    pen_mixed_yz<T, S>(P, o);
+}
+
+// MODE 0: q = A p with the weighted partial p.q.
+// MODE 1: reliable-update residual r = b - A x (FP64 in, r stored as R for flagged
+//         lanes) with the weighted partial r.r of the FP64 residual.
+template <typename T, typename R, int MODE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EQD_PENCIL_WAVES))) void k_pencil(
+    PencilView<T> op, int NN, const T* __restrict__ in, T* __restrict__ q, const T* __restrict__ b,
+    R* __restrict__ rout, const ColState* __restrict__ cs, const int* __restrict__ tile_on,
+    double* __restrict__ partial) {
+    constexpr int S = pencil_s<T>();
+    int tile, g;
+    tile_group(tile, g);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    extern __shared__ unsigned char eqd_lds[];
+    T* lds = reinterpret_cast<T*>(eqd_lds);
+    const int rows = op.nseg * S;
+    const int nS = rows * (2 * kRxS + 1), nO = rows * (2 * kRxO + 1);
+    for (int i = threadIdx.x; i < nS; i += kBlock) lds[i] = op.AXs[i];
+    for (int i = threadIdx.x; i < nO; i += kBlock) {
+        lds[nS + i] = op.AXo[i];
+        lds[nS + nO + i] = op.MX[i];
+    }
+    __syncthreads();
+    const XBands<T> xb{lds, lds + nS, lds + nS + nO};
+    double acc = 0.0;
+    if (tile_on[tile]) {
+        const bool mine = MODE == 0 || cs[(size_t)tile * 64 + lane].rel_flag != 0;
+        const size_t off = (size_t)tile * NN * 3 * 64;
+        const __amdgpu_buffer_rsrc_t rs = eqd_rsrc(in + off, (uint32_t)((size_t)NN * 3 * 64 * sizeof(T)));
+        const int np = op.NY * op.NY * op.nseg;
+        for (int pi = g * 4 + wv; pi < np; pi += kGroups * 4) {
+            const int pidx = uni(pi);
+            const int seg = pidx % op.nseg, yz = pidx / op.nseg, y = yz % op.NY, z = yz / op.NY;
+            const int m0 = seg * S;
+            T o[3][S];
+            pencil_apply<T, S>(op, xb, rs, m0, y, z, lane, o);
+            EQD_FENCE();
+            const T hyz = op.Hy[y] * op.Hy[z];
+#pragma unroll
+            for (int i = 0; i < S; ++i) {
+                const int m = m0 + i;
+                if (m < op.NX) {
+                    const T hw = -(op.Hx[m] * hyz);
+                    const size_t e = off + ((size_t)(m + op.NX * (y + op.NY * z)) * 3) * 64 + lane;
+                    double d = 0.0;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const T v = hw * o[c][i];
+                        if (MODE == 0) {
+                            q[e + c * 64] = v;
+                            d += (double)in[e + c * 64] * (double)v;
+                        } else {
+                            const double rv = (double)b[e + c * 64] - (double)v;
+                            if (mine) rout[e + c * 64] = (R)rv;
+                            d += rv * rv;
+                        }
+                    }
+                    acc += (m == op.mface ? 2.0 : 1.0) * d;
+                }
+            }
+        }
+    }
+    block_partial<64>(acc, partial, tile, g);
+}
+
+// Dynamic LDS bytes k_pencil needs for the x1 bands of a view.
+template <typename T>
+inline size_t pencil_lds_bytes(const PencilView<T>& op) {
+    const size_t rows = (size_t)op.nseg * pencil_s<T>();
+    return rows * ((2 * kRxS + 1) + 2 * (2 * kRxO + 1)) * sizeof(T);
+}
+
+}  // namespace eqd

@@ -98,6 +98,20 @@ HostTab to_tab(const std::vector<double>& M, int rows, int cols) {
 
 }  // namespace
 
+std::vector<double> build_band(const HostTab& t, int R) {
+    const int Wb = 2 * R + 1;
+    std::vector<double> b((size_t)t.nrows * Wb, 0.0);
+    for (int r = 0; r < t.nrows; ++r)
+        for (int k = 0; k < t.width[r]; ++k) {
+            const int d = t.start[r] + k - r;
+            const double c = t.c[(size_t)r * kTabW + k];
+            if (c == 0.0) continue;
+            if (d < -R || d > R) throw std::runtime_error("operator row exceeds its band reach");
+            b[(size_t)r * Wb + R + d] = c;
+        }
+    return b;
+}
+
 HostTables build_tables(int n1, int n23, double L_normal, double L_fault, double lambda, double mu) {
     HostTables T;
     T.n1 = n1;

@@ -131,18 +131,19 @@ def test_stiffness_matches_forward_solve():
     assert rel(st[:nf], d2) < 1e-8 and rel(st[nf:], d3) < 1e-8
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_batching_is_bitwise_invariant(precision):
+@pytest.mark.parametrize("precision,tile_k,name", [("fp64", 16, "cube11"), ("mixed", 16, "cube11"),
+                                                    ("fp64", 64, "ring"), ("mixed", 64, "ring")])
+def test_batching_is_bitwise_invariant(precision, tile_k, name):
     """Columns computed in one batch, in small batches, or as a subset are identical bits."""
-    fe, o = pair("cube11", precision=precision, tile_k=16)
+    fe, o = pair(name, precision=precision, tile_k=tile_k)
     K_all = eqd.fault_stiffness(fe)
     it_all = eqd.solver_report(fe.rs)["iterations"]
-    fe2, _ = pair("cube11", precision=precision, tile_k=16, max_cols_in_flight=32)
+    fe2, _ = pair(name, precision=precision, tile_k=tile_k, max_cols_in_flight=tile_k)
     K_small = eqd.fault_stiffness(fe2)
     assert np.array_equal(K_all, K_small)
     assert eqd.solver_report(fe2.rs)["iterations"] == it_all
     cols = [7, 3, o.nf + 11, 0]
-    fe3, _ = pair("cube11", precision=precision, tile_k=16)
+    fe3, _ = pair(name, precision=precision, tile_k=tile_k)
     assert np.array_equal(eqd.fault_stiffness(fe3, cols=cols), K_all[:, cols])
 
 
