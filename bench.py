@@ -57,6 +57,33 @@ def build_device(solver, Kbuf, nf2):
     _lib.check(_lib.lib().eqd_fault_stiffness_device(solver.handle, _lib.STIFFNESS_EXACT, None, 0, Kbuf.ptr, nf2))
 
 
+def copy_calibration(mib):
+    """A plain copy kernel of known size (reads and writes mib MiB per launch)."""
+    import torch
+    n = mib * 2**20 // 4
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "bytes_read_per_launch": n * 4, "bytes_written_per_launch": n * 4}))
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed counter summary (FETCH_SIZE +
+    WRITE_SIZE, calibrated against a plain copy), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", "traffic_dz50_exact_build.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return int(parts[2])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,7 +98,13 @@ def main():
     ap.add_argument("--cpu_cols", type=int, default=64)
     ap.add_argument("--no_cpu", action="store_true")
     ap.add_argument("--no_extras", action="store_true", help="only the timed mixed-precision steps")
+    ap.add_argument("--copy_calib_mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (3 times) and exit: the "
+                         "reference kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_calib_mib:
+        copy_calibration(args.copy_calib_mib)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the K build shards by columns "
               f"across GPUs, see DESIGN.md)", file=sys.stderr)
@@ -156,7 +189,7 @@ def main():
                         else f"k_apply_dot<float,{args.tile_k}>")
         out["roofline"] = {"bound": "hbm", "kernel": stencil_name,
                            "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                           "frac": round(achieved / HBM_PEAK, 4), "traffic": measured_traffic(stencil_name),
                            "avg_launch_us": round(avg_us, 2), "launches": st_n,
                            "bytes_per_launch": int(bytes_per_launch)}
         out["kernel_share"] = {n: round(v[0] / tot_ms, 4) for n, v in kt.items()}
