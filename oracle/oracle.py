@@ -173,3 +173,128 @@ def bp8_geometry(dz: float, L_fault: float, L_normal: float, l_f: float = 400.0,
     if n1 < 9 or n23 < 9:
         raise ValueError(f"n1={n1}, n23={n23}: SBP order 4 needs at least 9 points")
     return Geometry(n1, n23, L_normal, L_fault, lam, mu, l_f)
+
+
+# ---------------------------------------------------------------------------
+# BP8 time loop (bp8_oracle.h): rate-state Newton, A_p, RHS and the Tsit5 driver.
+
+BP8_PARAM_NAMES = ["mu", "nu", "rho", "c_s", "a", "b", "D_RS", "V_star", "f_star", "sigma0", "tau_init",
+                   "V_init", "V_zero", "Q0", "L_gauss", "L_fwid", "beta", "phi", "k", "viscosity", "alpha",
+                   "S_well", "r_well", "l_f", "dz", "t_off", "t_f", "sigma_min"]
+
+
+class _BP8Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in BP8_PARAM_NAMES]
+
+
+def _bp8_lib():
+    L = lib()
+    if not getattr(L, "_bp8_ready", False):
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        L.eqo_bp8_default_params.argtypes = [C.POINTER(_BP8Params)]
+        L.eqo_solve_slip_rate.restype = C.c_int
+        L.eqo_solve_slip_rate.argtypes = [C.c_double] * 10 + [C.c_double, C.c_int, C.c_double,
+                                                               C.POINTER(C.c_double)]
+        L.eqo_bp8_create.restype = C.c_void_p
+        L.eqo_bp8_create.argtypes = [C.POINTER(_BP8Params), dp, C.c_int, C.c_int, C.c_char_p, C.c_size_t]
+        L.eqo_bp8_destroy.argtypes = [C.c_void_p]
+        L.eqo_bp8_state_length.argtypes = [C.c_void_p]
+        L.eqo_bp8_initial_state.argtypes = [C.c_void_p, dp]
+        L.eqo_bp8_evaluate.argtypes = [C.c_void_p, dp, C.c_double, dp, dp, dp, dp, dp]
+        L.eqo_bp8_rhs.argtypes = [C.c_void_p, dp, C.c_double, dp]
+        L.eqo_bp8_Ap.argtypes = [C.c_void_p, dp]
+        L.eqo_bp8_source.argtypes = [C.c_void_p, dp, dp]
+        L.eqo_bp8_reset_cache.argtypes = [C.c_void_p]
+        i64p = C.POINTER(C.c_int64)
+        L.eqo_bp8_run.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp,
+                                  C.POINTER(C.c_int), i64p, i64p, i64p]
+        L._bp8_ready = True
+    return L
+
+
+def bp8_default_params() -> dict:
+    """BP8Params defaults (Table 1, src/BP8.jl:50-87) as a dict."""
+    p = _BP8Params()
+    _bp8_lib().eqo_bp8_default_params(C.byref(p))
+    return {n: getattr(p, n) for n in BP8_PARAM_NAMES}
+
+
+def solve_slip_rate(T, theta, sigma, eta, a, b, Dc, V_star, f_star, V0=1e-12, tol=1e-12, maxiter=50,
+                    maxstep=5.0):
+    """Newton in ln V (src/RateStateFriction.jl:77-96); raises RuntimeError on non-convergence."""
+    V = C.c_double()
+    rc = _bp8_lib().eqo_solve_slip_rate(T, theta, sigma, eta, a, b, Dc, V_star, f_star, V0, tol, maxiter,
+                                        maxstep, C.byref(V))
+    if rc:
+        raise RuntimeError("solve_slip_rate failed to converge")
+    return V.value
+
+
+class OracleBP8:
+    """BP8 model around a given K (build_model minus the K build, src/BP8.jl:184-258)."""
+
+    def __init__(self, K: np.ndarray, n2f: int, params: dict | None = None, injection: str = "gaussian"):
+        L = _bp8_lib()
+        p = bp8_default_params()
+        p.update(params or {})
+        self.params = p
+        cp = _BP8Params(*[p[n] for n in BP8_PARAM_NAMES])
+        self.n2f = n2f
+        self.nf = n2f * n2f
+        Kc = np.ascontiguousarray(np.asarray(K, dtype=np.float64).ravel(order="F"))
+        if Kc.size != (2 * self.nf) ** 2:
+            raise ValueError("K must be 2nf x 2nf")
+        err = C.create_string_buffer(512)
+        inj = {"gaussian": 0, "peaceman": 1}.get(injection, -1)
+        self._h = L.eqo_bp8_create(C.byref(cp), Kc, n2f, inj, err, 512)
+        if not self._h:
+            raise RuntimeError(err.value.decode())
+        self.n = L.eqo_bp8_state_length(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _bp8_lib().eqo_bp8_destroy(self._h)
+            self._h = None
+
+    def initial_state(self):
+        u = np.zeros(self.n)
+        _bp8_lib().eqo_bp8_initial_state(self._h, u)
+        return u
+
+    def evaluate(self, u, t=0.0):
+        nf = self.nf
+        out = [np.zeros(nf) for _ in range(5)]
+        if _bp8_lib().eqo_bp8_evaluate(self._h, np.ascontiguousarray(u, dtype=np.float64), t, *out):
+            raise RuntimeError("solve_slip_rate failed to converge")
+        return dict(zip(["V2", "V3", "Vmag", "tau2", "tau3"], out))
+
+    def rhs(self, u, t):
+        du = np.zeros(self.n)
+        if _bp8_lib().eqo_bp8_rhs(self._h, np.ascontiguousarray(u, dtype=np.float64), t, du):
+            raise RuntimeError("solve_slip_rate failed to converge")
+        return du
+
+    def Ap(self):
+        A = np.zeros(self.nf * self.nf)
+        _bp8_lib().eqo_bp8_Ap(self._h, A)
+        return A.reshape(self.nf, self.nf)
+
+    def source(self):
+        s, w = np.zeros(self.nf), np.zeros(self.nf)
+        _bp8_lib().eqo_bp8_source(self._h, s, w)
+        return s, w
+
+    def reset_cache(self):
+        _bp8_lib().eqo_bp8_reset_cache(self._h)
+
+    def run(self, t0, tf, saveat, reltol=1e-8):
+        nmax = int(np.floor((tf - t0) / saveat + 1e-9)) + 2
+        ts, us, vmax = np.zeros(nmax), np.zeros(nmax * self.n), np.zeros(nmax)
+        ns, na, nr, nrhs = C.c_int(), C.c_int64(), C.c_int64(), C.c_int64()
+        rc = _bp8_lib().eqo_bp8_run(self._h, t0, tf, saveat, reltol, nmax, ts, us, vmax, C.byref(ns), C.byref(na),
+                                    C.byref(nr), C.byref(nrhs))
+        if rc:
+            raise RuntimeError("solve_slip_rate failed to converge")
+        k = ns.value
+        return dict(t=ts[:k], u=us[:k * self.n].reshape(k, self.n), vmax=vmax[:k], naccept=na.value,
+                    nreject=nr.value, nrhs=nrhs.value)
