@@ -1,0 +1,939 @@
+// BP8 time loop on the device (C ABI in include/eqd.h).
+//
+// State u = [s2; s3; ln theta; p (; p_well)] (src/BP8.jl:262,362-364) for the nf fault
+// nodes of Omega_f. One right-hand-side evaluation is one kernel: a wavefront owns a
+// fault node i and computes
+//   dtau2_i, dtau3_i   rows i and nf+i of K times the slip (K kept row-major in HBM, so
+//                      both rows stream coalesced, 16 B per lane per load),
+//   V                  the rate-state Newton solve in ln V (src/RateStateFriction.jl:77-96),
+//   (A_p p)_i          the pore-pressure operator as two 1D SBP rows (src/PorePressure.jl:36-43),
+// and writes the four derivative entries of node i plus, for a Tsit5 stage, the same
+// entries of the next stage's input (which only need this node's earlier stage values).
+// A step is six stage kernels, a one-block controller (error norm, PI step control,
+// accept/reject, saveat bookkeeping) and a commit kernel (saveat interpolation, FSAL
+// copy, first stage input of the next step). Several steps are captured in one HIP
+// graph; the host replays it and polls the controller block between replays.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/eqd.h"
+#include "eqd_internal.h"
+#include "host_common.h"
+
+using namespace eqd;
+using namespace eqd::host;
+
+namespace eqd {
+namespace bp8 {
+
+// Tsitouras (2011) Tsit5 tableau and the embedded error weights (b - b_hat), as published.
+__constant__ double kA[8][7] = {
+    {0}, {0}, {0.161},
+    {-0.008480655492356989, 0.335480655492357},
+    {2.897153057105493, -6.359448489975075, 4.3622954328695815},
+    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525},
+    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383},
+    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081, 2.324710524099774}};
+__constant__ double kC[8] = {0, 0, 0.161, 0.327, 0.9, 0.9800255409045097, 1.0, 1.0};
+__constant__ double kBt[7] = {-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995,
+                              -0.1447110071732629,     0.5823571654525552,     -0.45808210592918697,
+                              1.0 / 66.0};
+constexpr double kA21 = 0.161;
+
+// Tsit5 free interpolant: u(t + theta dt) = uprev + dt sum_j b_j(theta) k_j.
+__device__ inline void interp_weights(double th, double b[7]) {
+    const double t2 = th * th;
+    b[0] = -1.0530884977290216 * th * (th - 1.3299890189751412) * (t2 - 1.4364028541716351 * th + 0.7139816917074209);
+    b[1] = 0.1017 * t2 * (t2 - 2.1966568338249754 * th + 1.2949852507374631);
+    b[2] = 2.490627285651252793 * t2 * (t2 - 2.38535645472061657 * th + 1.57803468208092486);
+    b[3] = -16.54810288924490272 * (th - 1.21712927295533244) * (th - 0.61620406037800089) * t2;
+    b[4] = 47.37952196281928122 * (th - 1.203071208372362603) * (th - 0.658047292653547382) * t2;
+    b[5] = -34.87065786149660974 * (th - 1.2) * (th - 0.666666666666666667) * t2;
+    b[6] = 2.5 * (th - 1) * (th - 0.6) * t2;
+}
+
+struct Consts {
+    int nf, n2f, n2, nstate, injection, well_cell;
+    double tau0_2, tau0_3, eta, a, b, Dc, Vs, fs, sigma0, sigma_min, alpha, D_RS;
+    double src_rate;  // q0 / (beta phi): Gaussian source amplitude while injecting
+    double t_off, Q0, WI, S_e, S_well;
+};
+
+// Integrator state, written only by the one-block controller (and the host at start).
+struct Ctl {
+    double t, dt, qold, q11, t0, tf, saveat, reltol, tstop;
+    double t_step, dt_step, t_new, EEst;
+    long long naccept, nreject, nrhs, nsaved, nsave_max, maxiters, iters;
+    int done, retcode, accepted, clipped, skip_commit, save_first, save_count, end_save;
+};
+
+struct Dev {
+    Consts c;
+    const double* KR;      // K row-major, 2nf x 2nf
+    const double* Lx;      // n2f x n2f row-major (alpha not applied)
+    const double* src;     // nf
+    const unsigned char* active;
+    double* V2;
+    double* V3;
+    double* Vm;
+    double* tau2;
+    double* tau3;
+    double* Vprev;
+    double* sig_low;
+    long long* floor_hits;
+    unsigned char* floor_mask;
+    int* fail;
+    double* k;             // 7 x nstate (k1 .. k7)
+    double* uprev;
+    double* unew;
+    double* partial;       // nf: per-node sum of squared scaled errors
+    Ctl* ctl;
+    double* saves;         // nsave_max rows of ldS (>= nstate, even: rows stay 16-byte aligned)
+    double* save_t;        // nsave_max
+    int ldS;
+};
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Newton in x = ln V on g(V) = eta V + sigma a asinh(V C / (2 V*)) - T (src/RateStateFriction.jl:77-96):
+// step clamped to +-5, converged when |dx| < 1e-12, at most 50 iterations.
+__device__ inline bool slip_rate(double T, double theta, double sig, const Consts& c, double V0, double& V) {
+    const double C = exp((c.fs + c.b * log(c.Vs * theta / c.Dc)) / c.a);
+    double x = log(V0);
+    for (int it = 0; it < 50; ++it) {
+        const double v = exp(x);
+        const double u = v / (2 * c.Vs) * C;
+        const double g = c.eta * v + sig * c.a * asinh(u) - T;
+        const double dg = c.eta * v + sig * c.a * u / sqrt(1 + u * u);
+        const double dx = fmin(fmax(-g / dg, -5.0), 5.0);
+        x += dx;
+        if (fabs(dx) < 1e-12) {
+            V = exp(x);
+            return true;
+        }
+    }
+    V = exp(x);
+    return false;
+}
+
+__device__ inline double abstol_of(const Consts& c, int e) {
+    return e < 2 * c.nf ? 1e-14 : (e < 3 * c.nf ? 1e-10 : 1e-3);
+}
+
+// One right-hand-side evaluation at state y (src/BP8.jl:355-433), a wavefront per node.
+// stage 0: plain rhs! into kout at time t_fixed.
+// stage 2..6: Tsit5 stage s of the current step (time t + c_s dt from the controller):
+//             kout = k_s, and ynext = the input of stage s+1 (u_new for s = 6).
+// stage 7:    k_7 = f(u_new) and the node's sum of squared scaled error estimates.
+__global__ __launch_bounds__(256) void k_bp8_stage(Dev d, int stage, const double* __restrict__ y,
+                                                   double* __restrict__ kout, double* __restrict__ ynext,
+                                                   double t_fixed) {
+    const Consts& c = d.c;
+    if (stage != 0 && d.ctl->done) return;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= c.nf) return;
+    const double dt = stage != 0 ? d.ctl->dt : 0.0;
+    const double t = stage != 0 ? d.ctl->t + kC[stage] * dt : t_fixed;
+
+    // dtau = K s: rows i and nf + i
+    const int nh = c.n2 >> 1;
+    const double2* r2 = reinterpret_cast<const double2*>(d.KR + (size_t)i * c.n2);
+    const double2* r3 = reinterpret_cast<const double2*>(d.KR + (size_t)(c.nf + i) * c.n2);
+    const double2* sv = reinterpret_cast<const double2*>(y);
+    double a2 = 0.0, a3 = 0.0;
+    for (int j = lane; j < nh; j += 64) {
+        const double2 s = sv[j], k2 = r2[j], k3 = r3[j];
+        a2 += k2.x * s.x + k2.y * s.y;
+        a3 += k3.x * s.x + k3.y * s.y;
+    }
+    const double dtau2 = wave_sum(a2), dtau3 = wave_sum(a3);
+
+    // A_p p at node i = (a, b): alpha (sum_c Lx[a][c] p[c + n b] + sum_c Lx[b][c] p[a + n c])
+    const int n = c.n2f, ia = i % n, ib = i / n;
+    const double* P = y + 3 * c.nf;
+    double ap = 0.0;
+    for (int q = lane; q < n; q += 64) ap += d.Lx[ia * n + q] * P[q + n * ib] + d.Lx[ib * n + q] * P[ia + n * q];
+    ap = c.alpha * wave_sum(ap);
+
+    // rate-state slip rate (every lane computes the same scalar path)
+    const double phi = y[2 * c.nf + i], p = P[i];
+    double v2 = 0.0, v3 = 0.0, vm = 0.0, t2, t3;
+    const double T2 = c.tau0_2 + dtau2, T3 = c.tau0_3 + dtau3;
+    if (!d.active[i]) {
+        t2 = T2;
+        t3 = T3;
+    } else {
+        const double sraw = c.sigma0 - p;
+        const double sig = fmax(sraw, c.sigma_min);
+        const double theta = exp(phi);
+        const double T = sqrt(T2 * T2 + T3 * T3);
+        if (T != 0.0) {
+            double V;
+            const bool ok = slip_rate(T, theta, sig, c, d.Vprev[i], V);
+            if (!ok && lane == 0) d.fail[0] = 1;
+            v2 = V * T2 / T;
+            v3 = V * T3 / T;
+        }
+        vm = sqrt(v2 * v2 + v3 * v3);
+        t2 = T2 - c.eta * v2;
+        t3 = T3 - c.eta * v3;
+        if (lane == 0) {
+            d.Vprev[i] = fmax(vm, 1e-30);
+            if (sraw < d.sig_low[i]) d.sig_low[i] = sraw;
+            if (sraw < c.sigma_min) {
+                d.floor_hits[i] += 1;
+                d.floor_mask[i] = 1;
+            }
+        }
+    }
+    double f[5];
+    f[0] = v2;
+    f[1] = v3;
+    f[2] = exp(-phi) - vm / c.D_RS;
+    f[3] = ap;
+    int ne = 4;
+    int idx[5] = {i, c.nf + i, 2 * c.nf + i, 3 * c.nf + i, 4 * c.nf};
+    if (c.injection == 0) {
+        f[3] += (t < c.t_off ? c.src_rate : 0.0) * d.src[i];
+    } else if (i == c.well_cell) {
+        const double pw = y[4 * c.nf];
+        const double transfer = c.WI * (pw - p);
+        f[3] += transfer / c.S_e;
+        f[4] = ((t < c.t_off ? c.Q0 : 0.0) - transfer) / c.S_well;
+        ne = 5;
+    }
+    if (lane == 0) {
+        d.V2[i] = v2;
+        d.V3[i] = v3;
+        d.Vm[i] = vm;
+        d.tau2[i] = t2;
+        d.tau3[i] = t3;
+    }
+    if (lane < ne) {
+        const int e = idx[lane];
+        double fe = f[0];
+#pragma unroll
+        for (int q = 1; q < 5; ++q)
+            if (lane == q) fe = f[q];
+        kout[e] = fe;
+        if (stage >= 2 && stage <= 6) {
+            double s = 0.0;
+            for (int j = 0; j < stage - 1; ++j) s += kA[stage + 1][j] * d.k[(size_t)j * c.nstate + e];
+            s += kA[stage + 1][stage - 1] * fe;
+            ynext[e] = d.uprev[e] + dt * s;
+        } else if (stage == 7) {
+            double s = 0.0;
+            for (int j = 0; j < 6; ++j) s += kBt[j] * d.k[(size_t)j * c.nstate + e];
+            s += kBt[6] * fe;
+            const double ut = dt * s;
+            const double sc = abstol_of(c, e) + fmax(fabs(d.uprev[e]), fabs(d.unew[e])) * d.ctl->reltol;
+            const double r = ut / sc;
+            double r2 = r * r;
+            // lanes 0..ne-1 hold the node's entries: sum them in entry order
+            double tot = 0.0;
+            for (int q = 0; q < ne; ++q) tot += __shfl(r2, q, 64);
+            if (lane == 0) d.partial[i] = tot;
+        }
+    }
+}
+
+// Controller (one block): RMS error norm, PI step control with the OrdinaryDiffEq defaults,
+// accept/reject, saveat bookkeeping and the next step size clipped to the next stop.
+__global__ __launch_bounds__(256) void k_bp8_control(Dev d) {
+    Ctl& s = *d.ctl;
+    __shared__ double red[256];
+    __shared__ int skip;
+    if (threadIdx.x == 0) {
+        skip = s.done;
+        if (s.done) s.skip_commit = 1;
+    }
+    __syncthreads();
+    if (skip) return;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < d.c.nf; i += 256) acc += d.partial[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double beta1 = 7.0 / 50.0, beta2 = 2.0 / 25.0, qmin = 0.2, qmax = 10.0, gamma = 0.9;
+    const double EEst = sqrt(red[0] / d.c.nstate);
+    s.EEst = EEst;
+    s.nrhs += 6;
+    s.iters += 1;
+    s.skip_commit = 0;
+    s.end_save = 0;
+    s.save_count = 0;
+    double q;
+    if (EEst == 0.0) {
+        q = 1.0 / qmax;
+    } else {
+        s.q11 = pow(EEst, beta1);
+        q = s.q11 / pow(s.qold, beta2);
+        q = fmax(1.0 / qmax, fmin(1.0 / qmin, q / gamma));
+    }
+    s.t_step = s.t;
+    s.dt_step = s.dt;
+    if (d.fail[0]) {
+        s.done = 1;
+        s.retcode = 2;
+        s.accepted = 0;
+        s.skip_commit = 1;
+        return;
+    }
+    if (EEst <= 1.0) {
+        const double tnew = s.clipped ? s.tstop : s.t + s.dt;
+        s.t_new = tnew;
+        s.accepted = 1;
+        s.naccept += 1;
+        long long cnt = 0;
+        while (s.nsaved + cnt < s.nsave_max && s.t0 + (double)(s.nsaved + cnt) * s.saveat <= tnew + 1e-9 * s.saveat)
+            ++cnt;
+        s.save_first = (int)s.nsaved;
+        s.save_count = (int)cnt;
+        s.nsaved += cnt;
+        s.t = tnew;
+        s.qold = fmax(EEst, 1e-4);
+        s.dt = s.dt / q;
+        if (s.t >= s.tf) {
+            // the end point is always saved (it is on the saveat grid in the usual case)
+            const double last = s.t0 + (double)(s.nsaved - 1) * s.saveat;
+            if (s.nsaved < s.nsave_max && fabs(last - s.tf) > 1e-9 * s.saveat) {
+                s.end_save = 1;
+                s.nsaved += 1;
+            }
+        }
+    } else {
+        s.accepted = 0;
+        s.nreject += 1;
+        s.dt = s.dt / fmin(1.0 / qmin, s.q11 / gamma);
+    }
+    if (s.t >= s.tf) {
+        s.done = 1;
+        s.retcode = 0;
+        return;
+    }
+    if (s.iters >= s.maxiters) {
+        s.done = 1;
+        s.retcode = 1;
+        return;
+    }
+    const double tstop = (d.c.t_off > s.t && d.c.t_off < s.tf) ? d.c.t_off : s.tf;
+    s.tstop = tstop;
+    s.clipped = s.dt >= tstop - s.t;
+    if (s.clipped) s.dt = tstop - s.t;
+}
+
+// Commit: saveat outputs of an accepted step (free interpolant, or u_new on a grid hit),
+// FSAL (uprev <- u_new, k1 <- k7), then the first stage input y2 = uprev + dt a21 k1.
+// init = 1 only builds y2 (start of integration).
+__global__ __launch_bounds__(256) void k_bp8_commit(Dev d, double* __restrict__ y2, int init) {
+    const Ctl& s = *d.ctl;
+    if (!init && s.skip_commit) return;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    const int ns = d.c.nstate;
+    if (e >= ns) return;
+    double* k1 = d.k;
+    double up = d.uprev[e];
+    if (!init && s.accepted) {
+        const double un = d.unew[e];
+        for (int q = 0; q < s.save_count; ++q) {
+            const long long id = s.save_first + q;
+            const double ts = s.t0 + (double)id * s.saveat;
+            double v;
+            if (fabs(ts - s.t_new) <= 1e-9 * s.saveat) {
+                v = un;
+            } else {
+                double b[7];
+                interp_weights((ts - s.t_step) / s.dt_step, b);
+                double acc = 0.0;
+                for (int j = 0; j < 7; ++j) acc += b[j] * d.k[(size_t)j * ns + e];
+                v = up + s.dt_step * acc;
+            }
+            d.saves[(size_t)id * d.ldS + e] = v;
+            if (e == 0) d.save_t[id] = ts;
+        }
+        if (s.end_save) {
+            d.saves[(size_t)(s.nsaved - 1) * d.ldS + e] = un;
+            if (e == 0) d.save_t[s.nsaved - 1] = s.tf;
+        }
+        up = un;
+        d.uprev[e] = un;
+        k1[e] = d.k[(size_t)6 * ns + e];
+    }
+    y2[e] = up + s.dt * kA21 * k1[e];
+}
+
+// Row-major copy of a column-major device K.
+__global__ void k_transpose(const double* __restrict__ K, long long ldK, int n, double* __restrict__ KR) {
+    __shared__ double tile[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int i = bx + threadIdx.x, j = by + r;  // element K(i, j) = K[i + j ldK]
+        if (i < n && j < n) tile[r][threadIdx.x] = K[(size_t)i + (size_t)j * ldK];
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int i = bx + r, j = by + threadIdx.x;  // KR[i n + j] = K(i, j)
+        if (i < n && j < n) KR[(size_t)i * n + j] = tile[threadIdx.x][r];
+    }
+}
+
+__global__ void k_max(const double* __restrict__ v, int n, double* __restrict__ out) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, v[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+__global__ void k_reset_cache(Dev d) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= d.c.nf) return;
+    d.Vprev[i] = 1e-12;
+    d.sig_low[i] = __builtin_huge_val();
+    d.floor_hits[i] = 0;
+    d.floor_mask[i] = 0;
+    d.V2[i] = d.V3[i] = d.Vm[i] = d.tau2[i] = d.tau3[i] = 0.0;
+    if (i == 0) d.fail[0] = 0;
+}
+
+}  // namespace bp8
+}  // namespace eqd
+
+using namespace eqd::bp8;
+
+struct eqd_bp8 {
+    eqd_bp8_params p{};
+    Consts c{};
+    std::vector<double> Lx_h, src_h, w_h;
+    std::vector<unsigned char> active_h;
+    DevMem KR, Lx, src, active, V2, V3, Vm, tau2, tau3, Vprev, sig_low, floor_hits, floor_mask, fail;
+    DevMem k, uprev, unew, ya, yb, partial, ctl, saves, save_t, scratch;
+    int64_t save_cap = 0, nsaved = 0;
+    hipStream_t st = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_steps = 8;
+    Ctl* h_ctl = nullptr;
+
+    ~eqd_bp8() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (h_ctl) (void)hipHostFree(h_ctl);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    Dev dev() const {
+        Dev d;
+        d.c = c;
+        d.KR = KR.as<double>();
+        d.Lx = Lx.as<double>();
+        d.src = src.as<double>();
+        d.active = active.as<unsigned char>();
+        d.V2 = V2.as<double>();
+        d.V3 = V3.as<double>();
+        d.Vm = Vm.as<double>();
+        d.tau2 = tau2.as<double>();
+        d.tau3 = tau3.as<double>();
+        d.Vprev = Vprev.as<double>();
+        d.sig_low = sig_low.as<double>();
+        d.floor_hits = floor_hits.as<long long>();
+        d.floor_mask = floor_mask.as<unsigned char>();
+        d.fail = fail.as<int>();
+        d.k = k.as<double>();
+        d.uprev = uprev.as<double>();
+        d.unew = unew.as<double>();
+        d.partial = partial.as<double>();
+        d.ctl = ctl.as<Ctl>();
+        d.saves = saves.as<double>();
+        d.save_t = save_t.as<double>();
+        d.ldS = ldS();
+        return d;
+    }
+    int node_blocks() const { return (c.nf + 3) / 4; }
+    int ldS() const { return (c.nstate + 1) & ~1; }
+    int state_blocks() const { return (c.nstate + 255) / 256; }
+};
+
+namespace {
+
+void init_model(eqd_bp8& m, const eqd_bp8_params& p, int n2f, int injection) {
+    if (injection != EQD_INJECTION_GAUSSIAN && injection != EQD_INJECTION_PEACEMAN)
+        throw Error(EQD_ERR_ARG, "injection must be :gaussian or :peaceman, got " + std::to_string(injection));
+    if (n2f < 9)
+        throw Error(EQD_ERR_GRID, "Omega_f spans only " + std::to_string(n2f) +
+                                      " nodes; the pore-pressure operator needs at least 9");
+    if (!(p.l_f > 0) || !(p.a > 0) || !(p.D_RS > 0) || !(p.c_s > 0))
+        throw Error(EQD_ERR_ARG, "l_f, a, D_RS and c_s must be positive");
+    m.p = p;
+    Consts& c = m.c;
+    c.n2f = n2f;
+    c.nf = n2f * n2f;
+    c.n2 = 2 * c.nf;
+    c.injection = injection;
+    c.nstate = 4 * c.nf + (injection == EQD_INJECTION_PEACEMAN ? 1 : 0);
+    const double h = 2.0 * p.l_f / (n2f - 1);
+    std::vector<double> H;
+    sbp4_neumann_1d(n2f, h, m.Lx_h, H);
+    m.src_h.resize(c.nf);
+    m.w_h.resize(c.nf);
+    m.active_h.assign(c.nf, 1);
+    double best = std::numeric_limits<double>::infinity();
+    c.well_cell = 0;
+    for (int b = 0; b < n2f; ++b)
+        for (int a = 0; a < n2f; ++a) {
+            const int i = a + n2f * b;
+            const double x = -p.l_f + a * h, yv = -p.l_f + b * h;
+            const double r2 = x * x + yv * yv;
+            m.src_h[i] = std::exp(-r2 / (2 * p.L_gauss * p.L_gauss)) / (2 * M_PI * p.L_gauss * p.L_gauss);
+            m.w_h[i] = H[a] * H[b];
+            if (a == 0 || a == n2f - 1 || b == 0 || b == n2f - 1) m.active_h[i] = 0;
+            if (r2 < best) {
+                best = r2;
+                c.well_cell = i;
+            }
+        }
+    const double vn = std::sqrt(p.V_init * p.V_init + p.V_zero * p.V_zero);
+    c.tau0_2 = p.tau_init * p.V_init / vn;
+    c.tau0_3 = p.tau_init * p.V_zero / vn;
+    c.eta = p.mu / (2 * p.c_s);
+    c.a = p.a;
+    c.b = p.b;
+    c.Dc = p.D_RS;
+    c.D_RS = p.D_RS;
+    c.Vs = p.V_star;
+    c.fs = p.f_star;
+    c.sigma0 = p.sigma0;
+    c.sigma_min = p.sigma_min;
+    c.alpha = p.alpha;
+    c.src_rate = (p.Q0 / p.L_fwid) / (p.beta * p.phi);
+    c.t_off = p.t_off;
+    c.Q0 = p.Q0;
+    c.S_well = p.S_well;
+    c.WI = 0.0;
+    if (injection == EQD_INJECTION_PEACEMAN) {
+        const double re = 0.268 * p.dz;
+        if (!(re > p.r_well)) throw Error(EQD_ERR_ARG, "Peaceman equivalent radius 0.268*dz must exceed r_well");
+        c.WI = 2 * M_PI * p.k * p.L_fwid / (p.viscosity * std::log(re / p.r_well));
+    }
+    c.S_e = p.dz * p.dz * p.L_fwid * p.phi * p.beta;
+
+    check_device();
+    HIPCHK(hipStreamCreateWithFlags(&m.st, hipStreamNonBlocking));
+    HIPCHK(hipHostMalloc((void**)&m.h_ctl, sizeof(Ctl), hipHostMallocDefault));
+    const size_t nf = c.nf, ns = c.nstate;
+    m.KR.alloc((size_t)c.n2 * c.n2 * 8);
+    m.Lx.alloc(m.Lx_h.size() * 8);
+    m.src.alloc(nf * 8);
+    m.active.alloc(nf);
+    for (DevMem* v : {&m.V2, &m.V3, &m.Vm, &m.tau2, &m.tau3, &m.Vprev, &m.sig_low, &m.partial}) v->alloc(nf * 8);
+    m.floor_hits.alloc(nf * 8);
+    m.floor_mask.alloc(nf);
+    m.fail.alloc(sizeof(int));
+    m.k.alloc(7 * ns * 8);
+    for (DevMem* v : {&m.uprev, &m.unew, &m.ya, &m.yb, &m.scratch}) v->alloc(ns * 8);
+    m.ctl.alloc(sizeof(Ctl));
+    HIPCHK(hipMemcpy(m.Lx.p, m.Lx_h.data(), m.Lx_h.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m.src.p, m.src_h.data(), nf * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m.active.p, m.active_h.data(), nf, hipMemcpyHostToDevice));
+    std::memset(m.h_ctl, 0, sizeof(Ctl));
+    HIPCHK(hipMemcpy(m.ctl.p, m.h_ctl, sizeof(Ctl), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_reset_cache, dim3((c.nf + 255) / 256), dim3(256), 0, m.st, m.dev());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(m.st));
+}
+
+void set_K_device(eqd_bp8& m, const double* K_dev, int64_t ldK) {
+    const int n = m.c.n2;
+    hipLaunchKernelGGL(k_transpose, dim3((n + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, m.st, K_dev, (long long)ldK,
+                       n, m.KR.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(m.st));
+}
+
+void initial_state(const eqd_bp8& m, double* u) {
+    const eqd_bp8_params& p = m.p;
+    std::fill(u, u + m.c.nstate, 0.0);
+    // initial_state_from_strength (src/RateStateFriction.jl:127-130) at tau_init, |V_init|
+    const double Vm = std::sqrt(p.V_init * p.V_init + p.V_zero * p.V_zero);
+    const double f = (p.tau_init - m.c.eta * Vm) / p.sigma0;
+    const double theta0 =
+        (p.D_RS / p.V_star) * std::exp((p.a * std::log(2 * p.V_star * std::sinh(f / p.a) / Vm) - p.f_star) / p.b);
+    for (int i = 0; i < m.c.nf; ++i) u[2 * m.c.nf + i] = std::log(theta0);
+}
+
+// One rhs! at a device state into a device output, then the Newton status.
+void rhs_device(eqd_bp8& m, const double* y_dev, double t, double* out_dev) {
+    Dev d = m.dev();
+    hipLaunchKernelGGL(k_bp8_stage, dim3(m.node_blocks()), dim3(256), 0, m.st, d, 0, y_dev, out_dev,
+                       (double*)nullptr, t);
+    HIPCHK(hipGetLastError());
+}
+
+void check_newton(eqd_bp8& m) {
+    int f = 0;
+    HIPCHK(hipMemcpyAsync(&f, m.fail.p, sizeof(int), hipMemcpyDeviceToHost, m.st));
+    HIPCHK(hipStreamSynchronize(m.st));
+    if (f) {
+        const int zero = 0;
+        HIPCHK(hipMemcpy(m.fail.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+        throw Error(EQD_ERR_NEWTON, "solve_slip_rate failed to converge");
+    }
+}
+
+double rms(const std::vector<double>& v) {
+    double s = 0.0;
+    for (double x : v) s += x * x;
+    return std::sqrt(s / (double)v.size());
+}
+
+void build_graph(eqd_bp8& m) {
+    if (m.exec) return;
+    Dev d = m.dev();
+    double* k = m.k.as<double>();
+    const size_t ns = m.c.nstate;
+    double* ya = m.ya.as<double>();
+    double* yb = m.yb.as<double>();
+    double* un = m.unew.as<double>();
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(m.st, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < m.graph_steps; ++s) {
+        // stage s reads y_s and writes y_{s+1}: y2 = ya, y3 = yb, y4 = ya, y5 = yb, y6 = ya, u_new
+        const double* in[8] = {nullptr, nullptr, ya, yb, ya, yb, ya, un};
+        double* out[8] = {nullptr, nullptr, yb, ya, yb, ya, un, nullptr};
+        for (int stage = 2; stage <= 7; ++stage)
+            hipLaunchKernelGGL(k_bp8_stage, dim3(m.node_blocks()), dim3(256), 0, m.st, d, stage, in[stage],
+                               k + (size_t)(stage - 1) * ns, out[stage], 0.0);
+        hipLaunchKernelGGL(k_bp8_control, dim3(1), dim3(256), 0, m.st, d);
+        hipLaunchKernelGGL(k_bp8_commit, dim3(m.state_blocks()), dim3(256), 0, m.st, d, ya, 0);
+    }
+    HIPCHK(hipStreamEndCapture(m.st, &g));
+    HIPCHK(hipGraphInstantiate(&m.exec, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+}
+
+void integrate(eqd_bp8& m, double t0, double tf, double saveat, double reltol, int64_t maxiters, int64_t nsave_max,
+               double* ts, double* us, eqd_bp8_stats* stats) {
+    if (!(tf > t0)) throw Error(EQD_ERR_ARG, "tspan must have tf > t0");
+    if (!(saveat > 0)) throw Error(EQD_ERR_ARG, "saveat must be positive");
+    if (!(reltol > 0)) throw Error(EQD_ERR_ARG, "reltol must be positive");
+    if (nsave_max < 1) throw Error(EQD_ERR_ARG, "nsave_max must be at least 1");
+    if (maxiters <= 0) maxiters = 100000;
+    const int ns = m.c.nstate;
+    if (nsave_max > m.save_cap) {
+        m.saves.alloc((size_t)nsave_max * m.ldS() * 8);
+        m.save_t.alloc((size_t)nsave_max * 8);
+        m.save_cap = nsave_max;
+        if (m.exec) {
+            HIPCHK(hipGraphExecDestroy(m.exec));
+            m.exec = nullptr;
+        }
+    }
+    m.nsaved = 0;
+    // u0, k1 = f(u0), Hairer's initial step (OrdinaryDiffEq ode_determine_initdt)
+    std::vector<double> u0(ns), f0(ns), f1(ns), u1(ns), tmp(ns), abst(ns);
+    initial_state(m, u0.data());
+    for (int e = 0; e < ns; ++e) abst[e] = e < 2 * m.c.nf ? 1e-14 : (e < 3 * m.c.nf ? 1e-10 : 1e-3);
+    HIPCHK(hipMemcpyAsync(m.uprev.p, u0.data(), ns * 8, hipMemcpyHostToDevice, m.st));
+    rhs_device(m, m.uprev.as<double>(), t0, m.k.as<double>());
+    HIPCHK(hipMemcpyAsync(f0.data(), m.k.p, ns * 8, hipMemcpyDeviceToHost, m.st));
+    check_newton(m);
+    std::vector<double> sk(ns);
+    for (int e = 0; e < ns; ++e) sk[e] = abst[e] + std::fabs(u0[e]) * reltol;
+    for (int e = 0; e < ns; ++e) tmp[e] = u0[e] / sk[e];
+    const double d0 = rms(tmp);
+    for (int e = 0; e < ns; ++e) tmp[e] = f0[e] / sk[e];
+    const double d1 = rms(tmp);
+    const double dtmax = tf - t0;
+    double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * (d0 / d1);
+    dt0 = std::min(dt0, dtmax);
+    for (int e = 0; e < ns; ++e) u1[e] = u0[e] + dt0 * f0[e];
+    HIPCHK(hipMemcpyAsync(m.ya.p, u1.data(), ns * 8, hipMemcpyHostToDevice, m.st));
+    rhs_device(m, m.ya.as<double>(), t0 + dt0, m.scratch.as<double>());
+    HIPCHK(hipMemcpyAsync(f1.data(), m.scratch.p, ns * 8, hipMemcpyDeviceToHost, m.st));
+    check_newton(m);
+    for (int e = 0; e < ns; ++e) tmp[e] = (f1[e] - f0[e]) / sk[e];
+    const double d2 = rms(tmp) / dt0;
+    const double md = std::max(d1, d2);
+    const double dt1 = md <= 1e-15 ? std::max(1e-6, dt0 * 1e-3) : std::pow(10.0, -(2 + std::log10(md)) / 5.0);
+    double dt = std::min(std::min(100 * dt0, dt1), dtmax);
+
+    Ctl& c = *m.h_ctl;
+    std::memset(&c, 0, sizeof(Ctl));
+    c.t = t0;
+    c.t0 = t0;
+    c.tf = tf;
+    c.saveat = saveat;
+    c.reltol = reltol;
+    c.qold = 1e-4;
+    c.nsave_max = nsave_max;
+    c.maxiters = maxiters;
+    c.nrhs = 2;
+    c.nsaved = 1;
+    c.tstop = (m.c.t_off > t0 && m.c.t_off < tf) ? m.c.t_off : tf;
+    c.clipped = dt >= c.tstop - t0;
+    c.dt = c.clipped ? c.tstop - t0 : dt;
+    HIPCHK(hipMemcpyAsync(m.ctl.p, &c, sizeof(Ctl), hipMemcpyHostToDevice, m.st));
+    HIPCHK(hipMemcpyAsync(m.saves.p, m.uprev.p, ns * 8, hipMemcpyDeviceToDevice, m.st));
+    HIPCHK(hipMemcpyAsync(m.save_t.p, &c.t0, 8, hipMemcpyHostToDevice, m.st));
+    hipLaunchKernelGGL(k_bp8_commit, dim3(m.state_blocks()), dim3(256), 0, m.st, m.dev(), m.ya.as<double>(), 1);
+    HIPCHK(hipGetLastError());
+    build_graph(m);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, m.st));
+    int64_t replays = 0;
+    const int64_t max_replays = maxiters / m.graph_steps + 2;
+    for (;;) {
+        HIPCHK(hipGraphLaunch(m.exec, m.st));
+        HIPCHK(hipMemcpyAsync(m.h_ctl, m.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, m.st));
+        HIPCHK(hipStreamSynchronize(m.st));
+        if (m.h_ctl->done) break;
+        if (++replays > max_replays) throw Error(EQD_ERR_DEVICE, "BP8 step loop did not terminate");
+    }
+    HIPCHK(hipEventRecord(e1, m.st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const Ctl& r = *m.h_ctl;
+    m.nsaved = r.nsaved;
+    if (stats) {
+        stats->naccept = r.naccept;
+        stats->nreject = r.nreject;
+        stats->nrhs = r.nrhs;
+        stats->nsaved = r.nsaved;
+        stats->retcode = r.retcode == 1 ? 1 : 0;
+        stats->ms = ms;
+    }
+    if (r.retcode == 2) {
+        const int zero = 0;
+        HIPCHK(hipMemcpy(m.fail.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+        throw Error(EQD_ERR_NEWTON, "solve_slip_rate failed to converge");
+    }
+    if (ts) HIPCHK(hipMemcpy(ts, m.save_t.p, (size_t)r.nsaved * 8, hipMemcpyDeviceToHost));
+    if (us)
+        HIPCHK(hipMemcpy2D(us, (size_t)ns * 8, m.saves.p, (size_t)m.ldS() * 8, (size_t)ns * 8, (size_t)r.nsaved,
+                           hipMemcpyDeviceToHost));
+}
+
+}  // namespace
+
+extern "C" {
+
+void eqd_bp8_default_params(eqd_bp8_params* p) {
+    if (!p) return;
+    // Table 1 (src/BP8.jl:50-87)
+    *p = eqd_bp8_params{32.04e9, 0.25, 2670.0, 3464.0, 0.016, 0.010, 0.5e-3, 1e-6, 0.6, 25.0e6, 14.6e6, 1e-12,
+                        1e-20, 0.003, 50.0, 1.0, 1e-8, 0.1, 5e-14, 1e-3, 0.05, 1e-7, 0.05, 400.0, 10.0,
+                        100 * 3600.0, 30 * 24 * 3600.0, 1.0e3};
+}
+
+eqd_status eqd_bp8_create(const eqd_bp8_params* p, const double* K, int64_t ldK, int n2f, int injection,
+                          eqd_bp8** out) {
+    return guard([&] {
+        if (!p || !K || !out) throw Error(EQD_ERR_ARG, "null argument");
+        auto m = std::make_unique<eqd_bp8>();
+        init_model(*m, *p, n2f, injection);
+        const int n = m->c.n2;
+        if (ldK < n) throw Error(EQD_ERR_ARG, "ldK must be at least 2nf");
+        std::vector<double> KR((size_t)n * n);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) KR[(size_t)i * n + j] = K[(size_t)i + (size_t)j * ldK];
+        HIPCHK(hipMemcpy(m->KR.p, KR.data(), KR.size() * 8, hipMemcpyHostToDevice));
+        *out = m.release();
+    });
+}
+
+eqd_status eqd_bp8_create_device(const eqd_bp8_params* p, const double* K_dev, int64_t ldK, int n2f, int injection,
+                                 eqd_bp8** out) {
+    return guard([&] {
+        if (!p || !K_dev || !out) throw Error(EQD_ERR_ARG, "null argument");
+        auto m = std::make_unique<eqd_bp8>();
+        init_model(*m, *p, n2f, injection);
+        if (ldK < m->c.n2) throw Error(EQD_ERR_ARG, "ldK must be at least 2nf");
+        set_K_device(*m, K_dev, ldK);
+        *out = m.release();
+    });
+}
+
+eqd_status eqd_bp8_create_from_solver(eqd_solver* s, int stiffness, const eqd_bp8_params* p, int injection,
+                                      eqd_bp8** out) {
+    return guard([&] {
+        if (!s || !p || !out) throw Error(EQD_ERR_ARG, "null argument");
+        if (stiffness != EQD_STIFFNESS_EXACT && stiffness != EQD_STIFFNESS_TOEPLITZ)
+            throw Error(EQD_ERR_ARG, "stiffness must be :exact or :toeplitz, got " + std::to_string(stiffness));
+        int nf = 0, n2f = 0, n3f = 0;
+        solver_fault_dims(s, nf, n2f, n3f);
+        if (n2f != n3f) throw Error(EQD_ERR_GRID, "Omega_f must be square (n2f == n3f)");
+        auto m = std::make_unique<eqd_bp8>();
+        init_model(*m, *p, n2f, injection);
+        DevMem K((size_t)2 * nf * 2 * nf * 8);
+        const eqd_status rc = eqd_fault_stiffness_device(s, stiffness, nullptr, 0, K.as<double>(), 2 * nf);
+        if (rc != EQD_OK) throw Error(rc, g_err);
+        set_K_device(*m, K.as<double>(), 2 * nf);
+        *out = m.release();
+    });
+}
+
+eqd_status eqd_bp8_destroy(eqd_bp8* m) {
+    return guard([&] { delete m; });
+}
+
+eqd_status eqd_bp8_sizes(const eqd_bp8* m, int* nf, int* state_len) {
+    return guard([&] {
+        if (!m) throw Error(EQD_ERR_ARG, "null model");
+        if (nf) *nf = m->c.nf;
+        if (state_len) *state_len = m->c.nstate;
+    });
+}
+
+eqd_status eqd_bp8_stiffness(const eqd_bp8* m, double* K, int64_t ldK) {
+    return guard([&] {
+        if (!m || !K) throw Error(EQD_ERR_ARG, "null argument");
+        const int n = m->c.n2;
+        if (ldK < n) throw Error(EQD_ERR_ARG, "ldK must be at least 2nf");
+        std::vector<double> KR((size_t)n * n);
+        HIPCHK(hipMemcpy(KR.data(), m->KR.p, KR.size() * 8, hipMemcpyDeviceToHost));
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) K[(size_t)i + (size_t)j * ldK] = KR[(size_t)i * n + j];
+    });
+}
+
+eqd_status eqd_bp8_operators(const eqd_bp8* m, double* Ap, double* source, double* weights) {
+    return guard([&] {
+        if (!m) throw Error(EQD_ERR_ARG, "null model");
+        const int n = m->c.n2f, nf = m->c.nf;
+        if (Ap) {
+            std::fill(Ap, Ap + (size_t)nf * nf, 0.0);
+            for (int b = 0; b < n; ++b)
+                for (int a = 0; a < n; ++a) {
+                    const size_t row = (size_t)(a + n * b) * nf;
+                    for (int q = 0; q < n; ++q) {
+                        Ap[row + q + n * b] += m->p.alpha * m->Lx_h[(size_t)a * n + q];
+                        Ap[row + a + n * q] += m->p.alpha * m->Lx_h[(size_t)b * n + q];
+                    }
+                }
+        }
+        if (source) std::copy(m->src_h.begin(), m->src_h.end(), source);
+        if (weights) std::copy(m->w_h.begin(), m->w_h.end(), weights);
+    });
+}
+
+eqd_status eqd_bp8_initial_state(const eqd_bp8* m, double* u) {
+    return guard([&] {
+        if (!m || !u) throw Error(EQD_ERR_ARG, "null argument");
+        initial_state(*m, u);
+    });
+}
+
+eqd_status eqd_bp8_evaluate(eqd_bp8* m, const double* u, double t, double* V2, double* V3, double* Vmag, double* tau2,
+                            double* tau3) {
+    return guard([&] {
+        if (!m || !u) throw Error(EQD_ERR_ARG, "null argument");
+        const int ns = m->c.nstate, nf = m->c.nf;
+        HIPCHK(hipMemcpyAsync(m->ya.p, u, ns * 8, hipMemcpyHostToDevice, m->st));
+        rhs_device(*m, m->ya.as<double>(), t, m->scratch.as<double>());
+        check_newton(*m);
+        if (V2) HIPCHK(hipMemcpy(V2, m->V2.p, nf * 8, hipMemcpyDeviceToHost));
+        if (V3) HIPCHK(hipMemcpy(V3, m->V3.p, nf * 8, hipMemcpyDeviceToHost));
+        if (Vmag) HIPCHK(hipMemcpy(Vmag, m->Vm.p, nf * 8, hipMemcpyDeviceToHost));
+        if (tau2) HIPCHK(hipMemcpy(tau2, m->tau2.p, nf * 8, hipMemcpyDeviceToHost));
+        if (tau3) HIPCHK(hipMemcpy(tau3, m->tau3.p, nf * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+eqd_status eqd_bp8_rhs(eqd_bp8* m, const double* u, double t, double* du) {
+    return guard([&] {
+        if (!m || !u || !du) throw Error(EQD_ERR_ARG, "null argument");
+        const int ns = m->c.nstate;
+        HIPCHK(hipMemcpyAsync(m->ya.p, u, ns * 8, hipMemcpyHostToDevice, m->st));
+        rhs_device(*m, m->ya.as<double>(), t, m->scratch.as<double>());
+        check_newton(*m);
+        HIPCHK(hipMemcpy(du, m->scratch.p, ns * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+eqd_status eqd_bp8_stress_report(const eqd_bp8* m, double* sigma_lowest, int64_t* floor_hits,
+                                 unsigned char* floor_mask) {
+    return guard([&] {
+        if (!m) throw Error(EQD_ERR_ARG, "null model");
+        const int nf = m->c.nf;
+        std::vector<double> sl(nf);
+        std::vector<long long> fh(nf);
+        HIPCHK(hipMemcpy(sl.data(), m->sig_low.p, nf * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(fh.data(), m->floor_hits.p, nf * 8, hipMemcpyDeviceToHost));
+        if (sigma_lowest) *sigma_lowest = *std::min_element(sl.begin(), sl.end());
+        if (floor_hits) {
+            long long s = 0;
+            for (long long v : fh) s += v;
+            *floor_hits = s;
+        }
+        if (floor_mask) HIPCHK(hipMemcpy(floor_mask, m->floor_mask.p, nf, hipMemcpyDeviceToHost));
+    });
+}
+
+eqd_status eqd_bp8_reset_cache(eqd_bp8* m) {
+    return guard([&] {
+        if (!m) throw Error(EQD_ERR_ARG, "null model");
+        hipLaunchKernelGGL(k_reset_cache, dim3((m->c.nf + 255) / 256), dim3(256), 0, m->st, m->dev());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(m->st));
+    });
+}
+
+eqd_status eqd_bp8_integrate(eqd_bp8* m, double t0, double tf, double saveat, double reltol, int64_t maxiters,
+                             int64_t nsave_max, double* ts, double* us, eqd_bp8_stats* stats) {
+    return guard([&] {
+        if (!m) throw Error(EQD_ERR_ARG, "null model");
+        integrate(*m, t0, tf, saveat, reltol, maxiters, nsave_max, ts, us, stats);
+    });
+}
+
+eqd_status eqd_bp8_saved_device(const eqd_bp8* m, const double** us_dev, int64_t* ld, int64_t* nsaved) {
+    return guard([&] {
+        if (!m) throw Error(EQD_ERR_ARG, "null model");
+        if (us_dev) *us_dev = m->saves.as<double>();
+        if (ld) *ld = m->ldS();
+        if (nsaved) *nsaved = m->nsaved;
+    });
+}
+
+eqd_status eqd_bp8_saved_vmax(eqd_bp8* m, double* vmax, int64_t n) {
+    return guard([&] {
+        if (!m || !vmax) throw Error(EQD_ERR_ARG, "null argument");
+        if (n > m->nsaved) throw Error(EQD_ERR_ARG, "more values requested than states saved");
+        if (n <= 0) return;
+        DevMem out((size_t)n * 8);
+        const size_t ld = m->ldS();
+        for (int64_t j = 0; j < n; ++j) {
+            rhs_device(*m, m->saves.as<double>() + (size_t)j * ld, 0.0, m->scratch.as<double>());
+            hipLaunchKernelGGL(k_max, dim3(1), dim3(256), 0, m->st, m->Vm.as<double>(), m->c.nf,
+                               out.as<double>() + j);
+        }
+        HIPCHK(hipGetLastError());
+        check_newton(*m);
+        HIPCHK(hipMemcpy(vmax, out.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"

@@ -17,73 +17,11 @@
 
 using namespace eqd;
 
+#include "host_common.h"
+
+using namespace eqd::host;
+
 namespace {
-
-thread_local std::string g_err;
-
-struct Error : std::runtime_error {
-    eqd_status code;
-    Error(eqd_status c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPCHK(expr)                                                                                 \
-    do {                                                                                             \
-        hipError_t e_ = (expr);                                                                      \
-        if (e_ != hipSuccess)                                                                        \
-            throw Error(e_ == hipErrorOutOfMemory ? EQD_ERR_NOMEM : EQD_ERR_DEVICE,                  \
-                        std::string(#expr) + ": " + hipGetErrorString(e_));                          \
-    } while (0)
-
-template <typename F>
-eqd_status guard(F&& f) {
-    try {
-        f();
-        return EQD_OK;
-    } catch (const Error& e) {
-        g_err = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_err = "host allocation failed";
-        return EQD_ERR_NOMEM;
-    } catch (const std::exception& e) {
-        g_err = e.what();
-        return EQD_ERR_ARG;
-    }
-}
-
-// Owning device allocation.
-struct DevMem {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevMem() = default;
-    explicit DevMem(size_t n) { alloc(n); }
-    void alloc(size_t n) {
-        release();
-        if (n) HIPCHK(hipMalloc(&p, n));
-        bytes = n;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    ~DevMem() { release(); }
-    DevMem(const DevMem&) = delete;
-    DevMem& operator=(const DevMem&) = delete;
-    template <typename T>
-    T* as() const { return static_cast<T*>(p); }
-};
-
-void check_device() {
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw Error(EQD_ERR_DEVICE, "no HIP device is visible");
-    int dev = 0;
-    HIPCHK(hipGetDevice(&dev));
-    hipDeviceProp_t prop;
-    HIPCHK(hipGetDeviceProperties(&prop, dev));
-    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-        throw Error(EQD_ERR_DEVICE, std::string("libeqd is built for gfx950, device is ") + prop.gcnArchName);
-}
 
 void validate_grid(const eqd_grid_desc& g) {
     if (g.n1 < 9 || g.n23 < 9)
@@ -305,6 +243,11 @@ struct eqd_solver {
     int64_t last_launches = 0, last_tile_iters = 0;
     std::vector<int> last_iters;  // per column of the last batch (slot order)
 };
+
+void eqd::host::solver_fault_dims(const eqd_solver* s, int& nf, int& n2f, int& n3f) {
+    nf = s->op->nf;
+    n2f = n3f = s->op->n2f;
+}
 
 namespace {
 

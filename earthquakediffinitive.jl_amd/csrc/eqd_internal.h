@@ -60,6 +60,10 @@ constexpr int kRyS = 4;  // Ts
 constexpr int kRyO = 2;  // To, D1, D1s
 std::vector<double> build_band(const HostTab& t, int R);
 
+// Pore-pressure operator factor along one Omega_f axis (D2 + Neumann SATs) and the
+// SBP-4 quadrature weights, n x n row-major.
+void sbp4_neumann_1d(int n, double h, std::vector<double>& L, std::vector<double>& H);
+
 // Device view of the band tables used by the pencil kernel.
 template <typename T>
 struct PencilView {

@@ -266,4 +266,17 @@ HostTables build_tables(int n1, int n23, double L_normal, double L_fault, double
     return T;
 }
 
+// One-dimensional factor of the pore-pressure operator on the Omega_f grid
+// (src/PorePressure.jl:36-43): the narrow D2 plus the Neumann (no-flux) SATs,
+// L = D2 + e_0 d_l / H_0 - e_N d_r / H_N, dense n x n row-major; H gets the weights.
+void sbp4_neumann_1d(int n, double h, std::vector<double>& L, std::vector<double>& H) {
+    Dense1D o = sbp4_dense(n, h);
+    L = o.D2;
+    for (int j = 0; j < n; ++j) {
+        L[j] += o.dl[j] / o.H[0];
+        L[(size_t)(n - 1) * n + j] -= o.dr[j] / o.H[n - 1];
+    }
+    H = o.H;
+}
+
 }  // namespace eqd

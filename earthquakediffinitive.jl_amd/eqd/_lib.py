@@ -27,7 +27,26 @@ EXPORTED = [
     "eqd_solver_duplicate", "eqd_solver_merge_stats", "eqd_solver_report", "eqd_split_node_solve",
     "eqd_shear_traction", "eqd_fault_stiffness", "eqd_fault_stiffness_device", "eqd_solver_last_timing", "eqd_solver_kernel_timing",
     "eqd_tables_export", "eqd_version", "eqd_last_error",
+    "eqd_bp8_default_params", "eqd_bp8_create", "eqd_bp8_create_device", "eqd_bp8_create_from_solver",
+    "eqd_bp8_destroy", "eqd_bp8_sizes", "eqd_bp8_stiffness", "eqd_bp8_operators", "eqd_bp8_initial_state",
+    "eqd_bp8_evaluate", "eqd_bp8_rhs", "eqd_bp8_stress_report", "eqd_bp8_reset_cache", "eqd_bp8_integrate",
+    "eqd_bp8_saved_device", "eqd_bp8_saved_vmax",
 ]
+
+INJECTION_GAUSSIAN, INJECTION_PEACEMAN = 0, 1
+
+BP8_PARAM_NAMES = ["mu", "nu", "rho", "c_s", "a", "b", "D_RS", "V_star", "f_star", "sigma0", "tau_init",
+                   "V_init", "V_zero", "Q0", "L_gauss", "L_fwid", "beta", "phi", "k", "viscosity", "alpha",
+                   "S_well", "r_well", "l_f", "dz", "t_off", "t_f", "sigma_min"]
+
+
+class BP8Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in BP8_PARAM_NAMES]
+
+
+class BP8Stats(C.Structure):
+    _fields_ = [("naccept", C.c_int64), ("nreject", C.c_int64), ("nrhs", C.c_int64), ("nsaved", C.c_int64),
+                ("retcode", C.c_int), ("ms", C.c_double)]
 
 
 class GridDesc(C.Structure):
@@ -93,6 +112,25 @@ def lib():
                                                C.POINTER(C.c_int64)]
         L.eqd_tables_export.argtypes = [C.POINTER(GridDesc), C.c_int, vp, vp, vp]
         L.eqd_tables_export.restype = C.c_int64
+        i64p = C.POINTER(C.c_int64)
+        L.eqd_bp8_default_params.argtypes = [C.POINTER(BP8Params)]
+        L.eqd_bp8_default_params.restype = None
+        L.eqd_bp8_create.argtypes = [C.POINTER(BP8Params), dp, C.c_int64, C.c_int, C.c_int, C.POINTER(vp)]
+        L.eqd_bp8_create_device.argtypes = [C.POINTER(BP8Params), vp, C.c_int64, C.c_int, C.c_int, C.POINTER(vp)]
+        L.eqd_bp8_create_from_solver.argtypes = [vp, C.c_int, C.POINTER(BP8Params), C.c_int, C.POINTER(vp)]
+        L.eqd_bp8_destroy.argtypes = [vp]
+        L.eqd_bp8_sizes.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.eqd_bp8_stiffness.argtypes = [vp, dp, C.c_int64]
+        L.eqd_bp8_operators.argtypes = [vp, vp, vp, vp]
+        L.eqd_bp8_initial_state.argtypes = [vp, dp]
+        L.eqd_bp8_evaluate.argtypes = [vp, dp, C.c_double, vp, vp, vp, vp, vp]
+        L.eqd_bp8_rhs.argtypes = [vp, dp, C.c_double, dp]
+        L.eqd_bp8_stress_report.argtypes = [vp, C.POINTER(C.c_double), i64p, vp]
+        L.eqd_bp8_reset_cache.argtypes = [vp]
+        L.eqd_bp8_integrate.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int64,
+                                        vp, vp, C.POINTER(BP8Stats)]
+        L.eqd_bp8_saved_device.argtypes = [vp, C.POINTER(vp), i64p, i64p]
+        L.eqd_bp8_saved_vmax.argtypes = [vp, dp, C.c_int64]
         L.eqd_version.restype = C.c_char_p
         L.eqd_last_error.argtypes = [C.c_char_p, C.c_size_t]
         L.eqd_last_error.restype = C.c_size_t

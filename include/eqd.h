@@ -144,6 +144,87 @@ eqd_status eqd_solver_kernel_timing(const eqd_solver* s, int kernel, double* tot
  * Any output pointer may be NULL. Returns the row count / vector length, or -1. */
 int64_t eqd_tables_export(const eqd_grid_desc* g, int which, int* start, int* width, double* coef);
 
+/* ------------------------------------------------------------------------------
+ * BP8 time loop on the device: K resident in HBM, one kernel per Tsit5 stage
+ * (K row GEMV + rate-state Newton + pore-pressure operator + stage combination),
+ * the step controller and the saveat interpolation on the device.
+ * ------------------------------------------------------------------------------ */
+typedef struct eqd_bp8 eqd_bp8;
+
+/* BP8Params (Table 1, src/BP8.jl:50-87); lambda = 2 mu nu / (1 - 2 nu), eta = mu / (2 c_s),
+ * q0 = Q0 / L_fwid are derived as build_model does. */
+typedef struct {
+    double mu, nu, rho, c_s;
+    double a, b, D_RS, V_star, f_star;
+    double sigma0, tau_init, V_init, V_zero;
+    double Q0, L_gauss, L_fwid, beta, phi, k, viscosity, alpha, S_well, r_well;
+    double l_f, dz, t_off, t_f, sigma_min;
+} eqd_bp8_params;
+
+enum { EQD_INJECTION_GAUSSIAN = 0, EQD_INJECTION_PEACEMAN = 1 };
+
+void eqd_bp8_default_params(eqd_bp8_params* p);
+
+/* build_model(...) minus the K build (src/BP8.jl:184-258): A_p, source, weights, active
+ * ring, tau0, Peaceman WI and cell volume around a given K (2nf x 2nf, column-major,
+ * leading dimension ldK; host memory for eqd_bp8_create, device memory for
+ * eqd_bp8_create_device). n2f = n3f = sqrt(nf). Errors as build_model: n2f < 9,
+ * unknown injection, r_e <= r_well. */
+eqd_status eqd_bp8_create(const eqd_bp8_params* p, const double* K, int64_t ldK, int n2f, int injection,
+                          eqd_bp8** out);
+eqd_status eqd_bp8_create_device(const eqd_bp8_params* p, const double* K_dev, int64_t ldK, int n2f,
+                                 int injection, eqd_bp8** out);
+/* build_model with the K build on the device (stiffness = EXACT or TOEPLITZ through the
+ * solver's engine); K never leaves device memory. */
+eqd_status eqd_bp8_create_from_solver(eqd_solver* s, int stiffness, const eqd_bp8_params* p, int injection,
+                                      eqd_bp8** out);
+eqd_status eqd_bp8_destroy(eqd_bp8* m);
+/* nf and the state length 4nf (+1 with the Peaceman well) (src/BP8.jl:262,362-364). */
+eqd_status eqd_bp8_sizes(const eqd_bp8* m, int* nf, int* state_len);
+/* The model's K (2nf x 2nf column-major, ldK >= 2nf) copied to the host. */
+eqd_status eqd_bp8_stiffness(const eqd_bp8* m, double* K, int64_t ldK);
+/* A_p as a dense nf x nf row-major matrix, the source and the quadrature weights (host). */
+eqd_status eqd_bp8_operators(const eqd_bp8* m, double* Ap, double* source, double* weights);
+/* initial_state(m) (src/BP8.jl:337-345): [0; 0; ln theta0; 0 (; 0)]. */
+eqd_status eqd_bp8_initial_state(const eqd_bp8* m, double* u);
+/* evaluate!(m, u, t) (src/BP8.jl:355-396): V2, V3, |V|, tau2, tau3 on Omega_f (any may be
+ * NULL). Like the reference it updates the model's Newton warm start and stress floor
+ * statistics. EQD_ERR_NEWTON if a node's Newton solve does not converge. */
+eqd_status eqd_bp8_evaluate(eqd_bp8* m, const double* u, double t, double* V2, double* V3, double* Vmag,
+                            double* tau2, double* tau3);
+/* rhs!(du, u, m, t) (src/BP8.jl:403-433). */
+eqd_status eqd_bp8_rhs(eqd_bp8* m, const double* u, double t, double* du);
+/* effective_stress_report(m) (src/BP8.jl:287): the smallest sigma0 - p seen before the
+ * floor, the number of floor hits (evaluations x nodes) and the mask of nodes ever floored
+ * (floor_mask[nf], may be NULL). */
+eqd_status eqd_bp8_stress_report(const eqd_bp8* m, double* sigma_lowest, int64_t* floor_hits,
+                                 unsigned char* floor_mask);
+/* Resets the evaluate! cache (warm start 1e-12, statistics), like a fresh Cache(nf). */
+eqd_status eqd_bp8_reset_cache(eqd_bp8* m);
+
+typedef struct {
+    int64_t naccept, nreject, nrhs;
+    int64_t nsaved;
+    int retcode;   /* 0 Success, 1 MaxIters (the step count reached maxiters) */
+    double ms;     /* device time of the step loop */
+} eqd_bp8_stats;
+
+/* run_bp8(m; tspan=(t0, tf), reltol, saveat) (src/BP8.jl:442-470): Tsit5 with the
+ * OrdinaryDiffEq defaults (PI control beta1 = 7/50, beta2 = 2/25, qmin 1/5, qmax 10,
+ * gamma 9/10; Hairer initial dt; RMS error norm; abstol 1e-14 / 1e-10 / 1e-3 (/ 1e-3) per
+ * block; tstops = [t_off]; save_everystep = false). States at t0, t0 + saveat, ... (and tf
+ * when it is on the grid) are written to ts[nsaved] and us[nsaved * state_len] (host,
+ * row per save; NULL keeps them on the device, see eqd_bp8_saved_device).
+ * maxiters <= 0 means 100000 (OrdinaryDiffEq's default). */
+eqd_status eqd_bp8_integrate(eqd_bp8* m, double t0, double tf, double saveat, double reltol, int64_t maxiters,
+                             int64_t nsave_max, double* ts, double* us, eqd_bp8_stats* stats);
+/* Device pointer to the saved states of the last integrate: row j (save j) starts at
+ * us_dev + j * ld (ld >= state_len, even). */
+eqd_status eqd_bp8_saved_device(const eqd_bp8* m, const double** us_dev, int64_t* ld, int64_t* nsaved);
+/* max_i |V_i| at every saved state of the last integrate, each evaluated with evaluate!
+ * in save order as write_outputs does (src/BP8.jl:628). */
+eqd_status eqd_bp8_saved_vmax(eqd_bp8* m, double* vmax, int64_t n);
+
 /* Version and last error. */
 const char* eqd_version(void);
 size_t eqd_last_error(char* buf, size_t len);

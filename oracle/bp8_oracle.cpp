@@ -376,15 +376,10 @@ int eqo_bp8_run(void* h, double t0, double tf, double saveat, double reltol, int
         const double beta1 = 7.0 / 50.0, beta2 = 2.0 / 25.0, qmin = 0.2, qmax = 10.0, gamma = 0.9;
         save(t, u.data());
         double next_save = t0 + saveat;
-        const std::vector<double> tstops = {m.p.t_off, tf};
         int64_t guard = 0;
         while (t < tf) {
-            double tstop = tf;
-            for (double s : tstops)
-                if (s > t) {
-                    tstop = s;
-                    break;
-                }
+            // tstops = [t_off]; a stop outside (t, tf) is ignored and tf always ends the run
+            const double tstop = (m.p.t_off > t && m.p.t_off < tf) ? m.p.t_off : tf;
             const bool clipped = dt >= tstop - t;
             if (clipped) dt = tstop - t;
             uprev = u;
@@ -437,6 +432,8 @@ int eqo_bp8_run(void* h, double t0, double tf, double saveat, double reltol, int
                     next_save = t0 + (saved) * saveat;
                 }
                 t = tnew;
+                // save_end: the final state is saved even when tf is off the saveat grid
+                if (t >= tf && saved < nsave_max && std::fabs(ts[saved - 1] - tf) > 1e-9 * saveat) save(tf, u.data());
                 std::swap(k[0], k[6]);
                 ++acc;
                 qold = std::max(EEst, 1e-4);
