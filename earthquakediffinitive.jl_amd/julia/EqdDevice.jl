@@ -1,0 +1,314 @@
+# EqdDevice: the Julia side of libeqd's C ABI (include/eqd.h).
+#
+# A drop-in for the hot path of EarthquakeDiffinitive.jl: the same function names and
+# argument meanings as ElasticitySplitNode (CGSolver, split_node_solve, duplicate,
+# merge_stats!, solver_report), FaultResponse (FaultElasticity, shear_traction,
+# fault_stiffness, fault_stiffness_toeplitz) and BP8 (build_model, evaluate!, rhs!,
+# run_bp8), executed on a gfx950 GPU. Arrays cross the boundary as plain pointers in
+# Julia's column-major Float64 layout, so `pointer(K)` is passed directly.
+#
+# Errors come back as an eqd_status; `check` rethrows them as ErrorException with the
+# library's message, as the reference's `error(...)` calls do. An unconverged CG solve
+# is counted and warned about once, not thrown (src/ElasticitySplitNode.jl:386-393).
+module EqdDevice
+
+export DeviceCGSolver, DeviceFaultElasticity, split_node_solve, duplicate, merge_stats!, solver_report,
+       shear_traction, fault_stiffness, fault_stiffness_toeplitz, frictional_node_count, fault_grid_axes,
+       DeviceBP8Model, build_model, initial_state, evaluate!, rhs!, run_bp8, state_length,
+       effective_stress_report
+
+const libeqd = joinpath(@__DIR__, "..", "libeqd.so")
+
+const EQD_OK = Cint(0)
+const EQD_PREC = Dict(:fp64 => Cint(0), :mixed => Cint(1))
+const EQD_PRECOND = Dict(:none => Cint(0), :jacobi => Cint(1))
+const EQD_ENGINE = Dict(:batched => Cint(0), :plain => Cint(1))
+const EQD_STIFFNESS = Dict(:exact => Cint(0), :toeplitz => Cint(1))
+const EQD_INJECTION = Dict(:gaussian => Cint(0), :peaceman => Cint(1))
+
+struct GridDesc
+    n1::Cint
+    n23::Cint
+    L_normal::Cdouble
+    L_fault::Cdouble
+    lambda::Cdouble
+    mu::Cdouble
+    l_f::Cdouble
+end
+
+struct OpInfo
+    nf::Int64
+    n2f::Int64
+    n3f::Int64
+    ntot::Int64
+    nreduced::Int64
+    h1::Cdouble
+    h2::Cdouble
+end
+
+mutable struct CgOpts
+    rtol::Cdouble
+    atol::Cdouble
+    itmax::Int64
+    precond::Cint
+    precision::Cint
+    engine::Cint
+    tile_k::Cint
+    max_cols_in_flight::Cint
+    reliable_interval::Cint
+    graph_iters::Cint
+    timing::Cint
+end
+CgOpts() = CgOpts(0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+
+mutable struct Report
+    solves::Int64
+    iterations::Int64
+    max_iterations::Int64
+    unconverged::Int64
+    mean_iterations::Cdouble
+    precond::Cint
+end
+Report() = Report(0, 0, 0, 0, 0.0, 0)
+
+function check(rc::Cint)
+    rc == EQD_OK && return nothing
+    buf = Vector{UInt8}(undef, 1024)
+    ccall((:eqd_last_error, libeqd), Csize_t, (Ptr{UInt8}, Csize_t), buf, length(buf))
+    error(unsafe_string(pointer(buf)))
+end
+
+# ---------------------------------------------------------------------------------
+# ElasticitySplitNode / FaultResponse (src/ElasticitySplitNode.jl:365-447,
+# src/FaultResponse.jl:96-393)
+
+mutable struct DeviceCGSolver
+    handle::Ptr{Cvoid}
+    op::Ptr{Cvoid}
+end
+
+mutable struct DeviceFaultElasticity
+    op::Ptr{Cvoid}
+    rs::DeviceCGSolver
+    info::OpInfo
+    x2f::Vector{Float64}
+    x3f::Vector{Float64}
+    Ntot::Int
+end
+
+function _solver(op::Ptr{Cvoid}; rtol=1e-10, atol=0.0, itmax=0, precond::Symbol=:none,
+                 precision::Symbol=:mixed, engine::Symbol=:batched, tile_k=0)
+    haskey(EQD_PRECOND, precond) || error("precond must be :none or :jacobi, got $precond")
+    o = CgOpts()
+    ccall((:eqd_cg_opts_default, libeqd), Cvoid, (Ref{CgOpts},), o)
+    o.rtol, o.atol, o.itmax = rtol, atol, itmax
+    o.precond, o.precision, o.engine, o.tile_k = EQD_PRECOND[precond], EQD_PREC[precision], EQD_ENGINE[engine], tile_k
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:eqd_solver_create, libeqd), Cint, (Ptr{Cvoid}, Ref{CgOpts}, Ref{Ptr{Cvoid}}), op, o, h))
+    s = DeviceCGSolver(h[], op)
+    finalizer(x -> ccall((:eqd_solver_destroy, libeqd), Cint, (Ptr{Cvoid},), x.handle), s)
+    return s
+end
+
+"""
+    DeviceFaultElasticity(n1, n23, L_normal, L_fault, λ, μ; l_f, solver_kwargs...)
+
+FaultElasticity(g₋, g₊, λ, μ, set; l_f, solver_kwargs...) for the two blocks
+[-L_normal,0]×[-L_fault,L_fault]² and [0,L_normal]×[...]² with n1×n23×n23 nodes each.
+"""
+function DeviceFaultElasticity(n1, n23, L_normal, L_fault, λ, μ; l_f, solver_kwargs...)
+    g = GridDesc(n1, n23, L_normal, L_fault, λ, μ, l_f)
+    op = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:eqd_operator_create, libeqd), Cint, (Ref{GridDesc}, Ref{Ptr{Cvoid}}), g, op))
+    info = Ref{OpInfo}()
+    check(ccall((:eqd_operator_info, libeqd), Cint, (Ptr{Cvoid}, Ref{OpInfo}), op[], info))
+    x2 = zeros(info[].n2f)
+    x3 = zeros(info[].n3f)
+    check(ccall((:eqd_fault_grid_axes, libeqd), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), op[], x2, x3))
+    fe = DeviceFaultElasticity(op[], _solver(op[]; solver_kwargs...), info[], x2, x3, Int(info[].ntot))
+    finalizer(x -> ccall((:eqd_operator_destroy, libeqd), Cint, (Ptr{Cvoid},), x.op), fe)
+    return fe
+end
+
+frictional_node_count(fe::DeviceFaultElasticity) = Int(fe.info.nf)
+fault_grid_axes(fe::DeviceFaultElasticity) = (fe.x2f, fe.x3f)
+
+function split_node_solve(s::DeviceCGSolver, rhs::Vector{Float64})
+    u = similar(rhs)
+    check(ccall((:eqd_split_node_solve, libeqd), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), s.handle, rhs, u))
+    return u
+end
+
+function duplicate(s::DeviceCGSolver)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:eqd_solver_duplicate, libeqd), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), s.handle, h))
+    d = DeviceCGSolver(h[], s.op)
+    finalizer(x -> ccall((:eqd_solver_destroy, libeqd), Cint, (Ptr{Cvoid},), x.handle), d)
+    return d
+end
+
+function merge_stats!(into::DeviceCGSolver, from::DeviceCGSolver)
+    check(ccall((:eqd_solver_merge_stats, libeqd), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), into.handle, from.handle))
+    return into
+end
+
+function solver_report(s::DeviceCGSolver)
+    r = Report()
+    check(ccall((:eqd_solver_report, libeqd), Cint, (Ptr{Cvoid}, Ref{Report}), s.handle, r))
+    return (; solves=r.solves, iterations=r.iterations, mean_iterations=r.mean_iterations,
+              max_iterations=r.max_iterations, unconverged=r.unconverged,
+              precond=r.precond == 0 ? :none : :jacobi)
+end
+
+function shear_traction(fe::DeviceFaultElasticity, s2, s3)
+    nf = frictional_node_count(fe)
+    Δτ2, Δτ3 = zeros(nf), zeros(nf)
+    check(ccall((:eqd_shear_traction, libeqd), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                fe.rs.handle, collect(Float64, s2), collect(Float64, s3), Δτ2, Δτ3))
+    return Δτ2, Δτ3
+end
+
+function _stiffness(fe::DeviceFaultElasticity, mode)
+    n = 2 * frictional_node_count(fe)
+    K = Matrix{Float64}(undef, n, n)
+    check(ccall((:eqd_fault_stiffness, libeqd), Cint, (Ptr{Cvoid}, Cint, Ptr{Cint}, Cint, Ptr{Float64}, Int64),
+                fe.rs.handle, mode, C_NULL, 0, K, n))
+    return K
+end
+
+"fault_stiffness(fe; verbose, threaded) (src/FaultResponse.jl:229): all 2nf columns on the device."
+fault_stiffness(fe::DeviceFaultElasticity; verbose=false, threaded=false) = _stiffness(fe, EQD_STIFFNESS[:exact])
+"fault_stiffness_toeplitz(fe; verbose) (src/FaultResponse.jl:339)."
+fault_stiffness_toeplitz(fe::DeviceFaultElasticity; verbose=false) = _stiffness(fe, EQD_STIFFNESS[:toeplitz])
+
+# ---------------------------------------------------------------------------------
+# BP8 (src/BP8.jl:184-470)
+
+struct BP8ParamsC
+    μ::Cdouble; ν::Cdouble; ρ::Cdouble; c_s::Cdouble
+    a::Cdouble; b::Cdouble; D_RS::Cdouble; V_star::Cdouble; f_star::Cdouble
+    σ0::Cdouble; τ_init::Cdouble; V_init::Cdouble; V_zero::Cdouble
+    Q0::Cdouble; L_gauss::Cdouble; L_fwid::Cdouble; β::Cdouble; φ::Cdouble; k::Cdouble
+    viscosity::Cdouble; α::Cdouble; S_well::Cdouble; r_well::Cdouble
+    l_f::Cdouble; Δz::Cdouble; t_off::Cdouble; t_f::Cdouble; σ̄_min::Cdouble
+end
+
+"Table 1 defaults, with keyword overrides by field name."
+function benchmark_parameters(; kw...)
+    p = Ref{BP8ParamsC}()
+    ccall((:eqd_bp8_default_params, libeqd), Cvoid, (Ref{BP8ParamsC},), p)
+    vals = Dict{Symbol,Float64}(f => getfield(p[], f) for f in fieldnames(BP8ParamsC))
+    for (k, v) in kw
+        haskey(vals, k) || error("unknown BP8 parameter $k")
+        vals[k] = v
+    end
+    return BP8ParamsC((vals[f] for f in fieldnames(BP8ParamsC))...)
+end
+
+mutable struct BP8Stats
+    naccept::Int64
+    nreject::Int64
+    nrhs::Int64
+    nsaved::Int64
+    retcode::Cint
+    ms::Cdouble
+end
+BP8Stats() = BP8Stats(0, 0, 0, 0, 0, 0.0)
+
+mutable struct DeviceBP8Model
+    handle::Ptr{Cvoid}
+    par::BP8ParamsC
+    nf::Int
+    state_len::Int
+    fe::Union{Nothing,DeviceFaultElasticity}
+end
+
+"""
+    build_model(; par, Δz, L_fault, L_normal, injection=:gaussian, stiffness=:toeplitz, solver_kwargs...)
+
+build_model (src/BP8.jl:184-258) with the K build on the device; K stays in device memory.
+"""
+function build_model(; par::BP8ParamsC=benchmark_parameters(), Δz=par.Δz, L_fault=3par.l_f,
+                     L_normal=2par.l_f, injection=:gaussian, order=4, verbose=false, stiffness=:toeplitz,
+                     solver_kwargs...)
+    injection ∈ (:gaussian, :peaceman) || error("injection must be :gaussian or :peaceman, got $injection")
+    isapprox(par.l_f / Δz, round(par.l_f / Δz); atol=1e-9) ||
+        error("Δz=$Δz must divide l_f=$(par.l_f) so the grid has nodes on ±l_f")
+    L_fault >= par.l_f || error("L_fault=$L_fault must be at least l_f=$(par.l_f)")
+    stiffness ∈ (:exact, :toeplitz) || error("stiffness must be :exact or :toeplitz, got $stiffness")
+    order == 4 || error("only SBP order 4 is provided")
+    n1 = round(Int, L_normal / Δz) + 1
+    n23 = round(Int, 2L_fault / Δz) + 1
+    n1 >= 9 || error("L_normal/Δz gives only $n1 points across the fault-normal direction; " *
+                     "SBP order 4 needs at least 9. Increase L_normal or decrease Δz.")
+    n23 >= 9 || error("2*L_fault/Δz gives only $n23 points along the fault; SBP order 4 needs at least 9.")
+    λ = 2par.μ * par.ν / (1 - 2par.ν)
+    fe = DeviceFaultElasticity(n1, n23, L_normal, L_fault, λ, par.μ; l_f=par.l_f, solver_kwargs...)
+    p = BP8ParamsC((f === :Δz ? Float64(Δz) : getfield(par, f) for f in fieldnames(BP8ParamsC))...)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:eqd_bp8_create_from_solver, libeqd), Cint,
+                (Ptr{Cvoid}, Cint, Ref{BP8ParamsC}, Cint, Ref{Ptr{Cvoid}}),
+                fe.rs.handle, EQD_STIFFNESS[stiffness], p, EQD_INJECTION[injection], h))
+    nf, ns = Ref{Cint}(0), Ref{Cint}(0)
+    check(ccall((:eqd_bp8_sizes, libeqd), Cint, (Ptr{Cvoid}, Ref{Cint}, Ref{Cint}), h[], nf, ns))
+    m = DeviceBP8Model(h[], p, nf[], ns[], fe)
+    finalizer(x -> ccall((:eqd_bp8_destroy, libeqd), Cint, (Ptr{Cvoid},), x.handle), m)
+    return m
+end
+
+state_length(m::DeviceBP8Model) = m.state_len
+
+function initial_state(m::DeviceBP8Model)
+    u = zeros(m.state_len)
+    check(ccall((:eqd_bp8_initial_state, libeqd), Cint, (Ptr{Cvoid}, Ptr{Float64}), m.handle, u))
+    return u
+end
+
+"evaluate!(m, u, t) (src/BP8.jl:355-396) -> (; V2, V3, Vmag, τ2, τ3)"
+function evaluate!(m::DeviceBP8Model, u::Vector{Float64}, t)
+    V2, V3, Vmag, τ2, τ3 = (zeros(m.nf) for _ in 1:5)
+    check(ccall((:eqd_bp8_evaluate, libeqd), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Float64}), m.handle, u, t, V2, V3, Vmag, τ2, τ3))
+    return (; V2, V3, Vmag, τ2, τ3)
+end
+
+"rhs!(du, u, m, t) (src/BP8.jl:403-433)"
+function rhs!(du::Vector{Float64}, u::Vector{Float64}, m::DeviceBP8Model, t)
+    check(ccall((:eqd_bp8_rhs, libeqd), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Ptr{Float64}),
+                m.handle, u, t, du))
+    return du
+end
+
+"""
+    run_bp8(m; tspan=(0.0, m.par.t_f), reltol=1e-8, saveat=3600.0, maxiters=100000) -> (; t, u, stats)
+
+run_bp8 (src/BP8.jl:442-470): Tsit5 with OrdinaryDiffEq's defaults, stepped on the device.
+"""
+function run_bp8(m::DeviceBP8Model; tspan=(0.0, m.par.t_f), reltol=1e-8, saveat=3600.0, verbose=false,
+                 maxiters=100000)
+    t0, tf = Float64.(tspan)
+    nmax = floor(Int, (tf - t0) / saveat + 1e-9) + 2
+    ts = zeros(nmax)
+    us = zeros(m.state_len, nmax)     # column per save == the C ABI's row per save
+    st = BP8Stats()
+    check(ccall((:eqd_bp8_integrate, libeqd), Cint,
+                (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Int64, Int64, Ptr{Float64}, Ptr{Float64},
+                 Ref{BP8Stats}), m.handle, t0, tf, saveat, reltol, maxiters, nmax, ts, us, st))
+    n = st.nsaved
+    return (; t=ts[1:n], u=[us[:, j] for j in 1:n], stats=(; naccept=st.naccept, nreject=st.nreject, nf=st.nrhs),
+              retcode=st.retcode == 0 ? :Success : :MaxIters)
+end
+
+function effective_stress_report(m::DeviceBP8Model)
+    lo, hits = Ref{Cdouble}(0), Ref{Int64}(0)
+    mask = zeros(UInt8, m.nf)
+    check(ccall((:eqd_bp8_stress_report, libeqd), Cint, (Ptr{Cvoid}, Ref{Cdouble}, Ref{Int64}, Ptr{UInt8}),
+                m.handle, lo, hits, mask))
+    nodes = count(!iszero, mask)
+    return (; σ̄_lowest=lo[], floor_hits=hits[], floor=m.par.σ̄_min, bound=hits[] > 0, nodes,
+              fraction=nodes / m.nf)
+end
+
+end # module
