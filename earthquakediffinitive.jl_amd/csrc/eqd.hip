@@ -314,8 +314,12 @@ struct Engine {
             if constexpr (K == 64) {
                 PencilView<T> pv;
                 if constexpr (std::is_same<T, float>::value) pv = op.p32; else pv = op.p64;
-                hipLaunchKernelGGL((k_pencil<T, T, 0>), grid, dim3(kBlock), pencil_lds_bytes(pv), st, pv, op.NN, p, q,
-                                   (const T*)nullptr, (T*)nullptr, cs, on, part);
+                if (pv.nseg == 1)
+                    hipLaunchKernelGGL((k_pencil<T, T, 0, true>), grid, dim3(kBlock), pencil_lds_bytes(pv), st, pv,
+                                       op.NN, p, q, (const T*)nullptr, (T*)nullptr, cs, on, part);
+                else
+                    hipLaunchKernelGGL((k_pencil<T, T, 0, false>), grid, dim3(kBlock), pencil_lds_bytes(pv), st, pv,
+                                       op.NN, p, q, (const T*)nullptr, (T*)nullptr, cs, on, part);
             } else {
                 hipLaunchKernelGGL((k_apply_dot<T, K>), grid, dim3(kBlock), 0, st, v, op.NN, p, q, on, part);
             }
@@ -325,18 +329,26 @@ struct Engine {
                                nullptr);
         });
         timed(1, [&] {
-            if constexpr (std::is_same<T, float>::value)
-                hipLaunchKernelGGL((k_update<T, float, K>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface,
-                                   w.xc.as<float>(), r, p, q, cs, on, part);
+            T* xu = nullptr;
+            if constexpr (std::is_same<T, float>::value) xu = w.xc.as<float>();
+            else xu = w.x.as<double>();
+            if constexpr (K == 64)
+                hipLaunchKernelGGL((k_update64<T>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface, xu, r, p,
+                                   q, cs, on, part);
             else
-                hipLaunchKernelGGL((k_update<T, double, K>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface,
-                                   w.x.as<double>(), r, p, q, cs, on, part);
+                hipLaunchKernelGGL((k_update<T, T, K>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface, xu,
+                                   r, p, q, cs, on, part);
         });
         timed(3, [&] {
             hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, 1, (int)mixed, R, cs, part, on, rel,
                                nullptr);
         });
-        timed(2, [&] { hipLaunchKernelGGL((k_pupdate<T, K>), grid, dim3(kBlock), 0, st, op.NN, p, r, cs, on); });
+        timed(2, [&] {
+            if constexpr (K == 64)
+                hipLaunchKernelGGL((k_pupdate64<T>), grid, dim3(kBlock), 0, st, op.NN, p, r, cs, on);
+            else
+                hipLaunchKernelGGL((k_pupdate<T, K>), grid, dim3(kBlock), 0, st, op.NN, p, r, cs, on);
+        });
         if (mixed) {
             timed(4, [&] {
                 hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, op.NN, w.x.as<double>(), w.xc.as<float>(),
@@ -344,10 +356,18 @@ struct Engine {
             });
             timed(5, [&] {
                 if constexpr (K == 64)
-                    hipLaunchKernelGGL((k_pencil<double, T, 1>), grid, dim3(kBlock), pencil_lds_bytes(op.p64), st,
-                                       op.p64, op.NN,
-                                       (const double*)w.x.as<double>(), (double*)nullptr,
-                                       (const double*)w.b.as<double>(), r, cs, rel, part);
+                {
+                    const double* x64 = w.x.as<double>();
+                    const double* b64 = w.b.as<double>();
+                    if (op.p64.nseg == 1)
+                        hipLaunchKernelGGL((k_pencil<double, T, 1, true>), grid, dim3(kBlock),
+                                           pencil_lds_bytes(op.p64), st, op.p64, op.NN, x64, (double*)nullptr, b64, r,
+                                           cs, rel, part);
+                    else
+                        hipLaunchKernelGGL((k_pencil<double, T, 1, false>), grid, dim3(kBlock),
+                                           pencil_lds_bytes(op.p64), st, op.p64, op.NN, x64, (double*)nullptr, b64, r,
+                                           cs, rel, part);
+                }
                 else
                     hipLaunchKernelGGL((k_residual<T, K>), grid, dim3(kBlock), 0, st, op.v64, op.NN, w.x.as<double>(),
                                        w.b.as<double>(), r, cs, rel, part);
@@ -610,10 +630,8 @@ void run_columns(eqd_solver* s, const std::vector<int64_t>& colid, const std::ve
         per = (int)std::max<size_t>(1, std::min<size_t>((size_t)per, cap));
     }
     if (s->o.engine == EQD_ENGINE_PLAIN) per = 1;
-    int ntiles = (std::min(per, n) + K - 1) / K;
-    // a multiple of 8 tiles lets the kernels deal tiles to the 8 XCDs (tile_group);
-    // the extra tiles hold idle columns and are skipped
-    if (ntiles > 1) ntiles = (ntiles + 7) / 8 * 8;
+    // tile_group balances any tile count over the 8 XCDs, so no padding tiles
+    const int ntiles = (std::min(per, n) + K - 1) / K;
     ensure_ws(s, K, ntiles, prec);
     Workspace& w = *s->ws;
     const int slots = ntiles * K;

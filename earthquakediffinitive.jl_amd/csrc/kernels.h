@@ -33,16 +33,25 @@ struct ColState {
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// (tile, group) of this block for the node-parallel grid (kGroups x ntiles). Blocks are
-// dispatched round-robin over the 8 XCDs in linear order; when ntiles is a multiple of
-// 8 the tiles are dealt to XCDs the same way, so each XCD's L2 (and its share of the
-// Infinity Cache) only ever holds its own tiles' vectors.
+// Block -> (tile, group) map. Workgroups are dispatched round-robin over the 8 XCDs in
+// linear block order, so XCD x receives blocks b = x, x + 8, ... With T tiles of kGroups
+// groups, the first 8*floor(T/8) tiles are dealt whole to XCDs (each XCD's L2 and its
+// share of the Infinity Cache only hold its own tiles' vectors) and the groups of the
+// remaining T % 8 tiles are spread evenly over all 8 XCDs, so every XCD carries T/8
+// tiles' work instead of some carrying one tile more.
 __device__ __forceinline__ void tile_group(int& tile, int& g) {
-    if ((gridDim.y & 7) == 0 && gridDim.x == kGroups) {
+    if (gridDim.x == kGroups) {
         const int b = blockIdx.x + gridDim.x * blockIdx.y;
         const int xcd = b & 7, q = b >> 3;
-        tile = ((q / kGroups) << 3) + xcd;
-        g = q % kGroups;
+        const int whole = (int)(gridDim.y >> 3) * kGroups;  // per-XCD blocks of the dealt tiles
+        if (q < whole) {
+            tile = ((q / kGroups) << 3) + xcd;
+            g = q % kGroups;
+        } else {
+            const int r = ((q - whole) << 3) + xcd;  // 0 .. (T % 8) * kGroups - 1
+            tile = (int)(gridDim.y & ~7u) + r / kGroups;
+            g = r % kGroups;
+        }
     } else {
         tile = blockIdx.y;
         g = blockIdx.x;
@@ -282,6 +291,139 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(int NN, T* __restrict__ p, c
         const size_t e = off + (size_t)node * 3 * K + lane;
 #pragma unroll
         for (int c = 0; c < 3; ++c) p[e + c * K] = r[e + c * K] + b * p[e + c * K];
+    }
+}
+
+// 16-byte vectors of the tile-interleaved layout for K = 64: a thread owns VW adjacent
+// columns of one (node, component) row, 64 / VW threads cover a row.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    using type = float4;
+    static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+    using type = double2;
+    static constexpr int n = 2;
+};
+template <typename T>
+__device__ __forceinline__ T vget(const typename Vec16<T>::type& v, int j) {
+    if constexpr (Vec16<T>::n == 4) return j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w));
+    else return j == 0 ? v.x : v.y;
+}
+template <typename T>
+__device__ __forceinline__ void vset(typename Vec16<T>::type& v, int j, T a) {
+    if constexpr (Vec16<T>::n == 4) {
+        if (j == 0) v.x = a;
+        else if (j == 1) v.y = a;
+        else if (j == 2) v.z = a;
+        else v.w = a;
+    } else {
+        if (j == 0) v.x = a;
+        else v.y = a;
+    }
+}
+
+// Per-column partials of the VW columns a thread owns, reduced over the block's rows in
+// a fixed order into partial[tile][g][64].
+template <int VW>
+__device__ __forceinline__ void block_partial_vec(const double (&acc)[VW], double* __restrict__ partial, int tile,
+                                                  int g) {
+    constexpr int TPR = 64 / VW, ROWS = kBlock / TPR;
+    __shared__ double sm[ROWS][64];
+    const int t = threadIdx.x, c0 = (t % TPR) * VW;
+#pragma unroll
+    for (int j = 0; j < VW; ++j) sm[t / TPR][c0 + j] = acc[j];
+    __syncthreads();
+    if (t < 64) {
+        double s = 0.0;
+        for (int i = 0; i < ROWS; ++i) s += sm[i][t];
+        partial[((size_t)tile * kGroups + g) * 64 + t] = s;
+    }
+}
+
+// x += alpha p, r -= alpha q and the weighted partial r.r, for 64-column tiles with
+// 16-byte accesses (the generic k_update moves 4 or 8 bytes per lane).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_update64(int NN, int NX, int mface, T* __restrict__ x, T* __restrict__ r,
+                                                     const T* __restrict__ p, const T* __restrict__ q,
+                                                     const ColState* __restrict__ cs, const int* __restrict__ tile_on,
+                                                     double* __restrict__ partial) {
+    using V = typename Vec16<T>::type;
+    constexpr int VW = Vec16<T>::n, TPR = 64 / VW;
+    int tile, g;
+    tile_group(tile, g);
+    const int t = threadIdx.x, c0 = (t % TPR) * VW;
+    double acc[VW];
+#pragma unroll
+    for (int j = 0; j < VW; ++j) acc[j] = 0.0;
+    if (tile_on[tile]) {
+        T a[VW];
+        bool act[VW];
+#pragma unroll
+        for (int j = 0; j < VW; ++j) {
+            const ColState& st = cs[(size_t)tile * 64 + c0 + j];
+            act[j] = st.status == kActive;
+            a[j] = (T)st.alpha;
+        }
+        const size_t off = (size_t)tile * NN * 192;
+        const size_t nv = (size_t)NN * 192 / VW;
+        V* xv = reinterpret_cast<V*>(x + off);
+        V* rv = reinterpret_cast<V*>(r + off);
+        const V* pv = reinterpret_cast<const V*>(p + off);
+        const V* qv = reinterpret_cast<const V*>(q + off);
+        for (size_t v = (size_t)g * kBlock + t; v < nv; v += (size_t)kGroups * kBlock) {
+            const int node = (int)(v * VW / 192);
+            const double w = (node % NX == mface) ? 2.0 : 1.0;
+            V xx = xv[v], rr = rv[v];
+            const V pp = pv[v], qq = qv[v];
+#pragma unroll
+            for (int j = 0; j < VW; ++j) {
+                if (act[j]) {
+                    vset<T>(xx, j, vget<T>(xx, j) + a[j] * vget<T>(pp, j));
+                    const T rn = vget<T>(rr, j) - a[j] * vget<T>(qq, j);
+                    vset<T>(rr, j, rn);
+                    acc[j] += w * ((double)rn * (double)rn);
+                }
+            }
+            xv[v] = xx;
+            rv[v] = rr;
+        }
+    }
+    block_partial_vec<VW>(acc, partial, tile, g);
+}
+
+// p = r + beta p for 64-column tiles with 16-byte accesses.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pupdate64(int NN, T* __restrict__ p, const T* __restrict__ r,
+                                                      const ColState* __restrict__ cs, const int* __restrict__ tile_on) {
+    using V = typename Vec16<T>::type;
+    constexpr int VW = Vec16<T>::n, TPR = 64 / VW;
+    int tile, g;
+    tile_group(tile, g);
+    if (!tile_on[tile]) return;
+    const int t = threadIdx.x, c0 = (t % TPR) * VW;
+    T b[VW];
+    bool act[VW];
+#pragma unroll
+    for (int j = 0; j < VW; ++j) {
+        const ColState& st = cs[(size_t)tile * 64 + c0 + j];
+        act[j] = st.status == kActive;
+        b[j] = (T)st.beta;
+    }
+    const size_t off = (size_t)tile * NN * 192;
+    const size_t nv = (size_t)NN * 192 / VW;
+    V* pv = reinterpret_cast<V*>(p + off);
+    const V* rv = reinterpret_cast<const V*>(r + off);
+    for (size_t v = (size_t)g * kBlock + t; v < nv; v += (size_t)kGroups * kBlock) {
+        V pp = pv[v];
+        const V rr = rv[v];
+#pragma unroll
+        for (int j = 0; j < VW; ++j)
+            if (act[j]) vset<T>(pp, j, vget<T>(rr, j) + b[j] * vget<T>(pp, j));
+        pv[v] = pp;
     }
 }
 

@@ -62,7 +62,9 @@ struct XBands {
     const T* MX;
 };
 
-template <typename T, int S>
+// F ("full line"): one segment covers the whole x1 line (NX <= S), so the x1 windows need
+// no halo: columns outside [0, NX) only ever meet zero band coefficients.
+template <typename T, int S, bool F>
 struct PencilCtx {
     const PencilView<T>& op;
     const XBands<T>& xb;
@@ -79,10 +81,10 @@ struct PencilCtx {
 };
 
 // x1 term of component Cc: AXs for component 0, AXo for 1 and 2.
-template <typename T, int S, int Cc>
-__device__ __forceinline__ void pen_x(const PencilCtx<T, S>& P, T (&o)[3][S]) {
+template <typename T, int S, bool F, int Cc>
+__device__ __forceinline__ void pen_x(const PencilCtx<T, S, F>& P, T (&o)[3][S]) {
     constexpr int RB = Cc == 0 ? kRxS : kRxO;
-    constexpr int R = kRxS, W = S + 2 * R;
+    constexpr int R = F ? 0 : kRxS, W = S + 2 * R;
     EQD_FENCE();
     const int L = P.line(P.y, P.z, Cc);
     T w[W];
@@ -97,7 +99,8 @@ __device__ __forceinline__ void pen_x(const PencilCtx<T, S>& P, T (&o)[3][S]) {
             const T* b = band + (size_t)(P.m0 + i) * (2 * RB + 1);
             T a = T(0);
 #pragma unroll
-            for (int d = -RB; d <= RB; ++d) a += b[RB + d] * w[i + R + d];
+            for (int d = -RB; d <= RB; ++d)
+                if (i + R + d >= 0 && i + R + d < W) a += b[RB + d] * w[i + R + d];
             o[Cc][i] += a;
         }
     }
@@ -117,9 +120,9 @@ __device__ __forceinline__ T ucoef(const T* p) {
 __device__ __forceinline__ int rclamp(int r, int n) { return r < 0 ? 0 : (r >= n ? n - 1 : r); }
 
 // Mixed x1-x2 (AX = 0) or x1-x3 (AX = 1): o[DST] += MX (D1 along AX of component SRC).
-template <typename T, int S, int AX, int SRC, int DST>
-__device__ __forceinline__ void pen_mixed_x(const PencilCtx<T, S>& P, T (&o)[3][S]) {
-    constexpr int R = kRxO, W = S + 2 * R;
+template <typename T, int S, bool F, int AX, int SRC, int DST>
+__device__ __forceinline__ void pen_mixed_x(const PencilCtx<T, S, F>& P, T (&o)[3][S]) {
+    constexpr int R = F ? 0 : kRxO, W = S + 2 * R;
     EQD_FENCE();
     T a[W];
 #pragma unroll
@@ -129,14 +132,16 @@ __device__ __forceinline__ void pen_mixed_x(const PencilCtx<T, S>& P, T (&o)[3][
 #pragma unroll EQD_LINE_UNROLL
     for (int d = -kRyO; d <= kRyO; ++d) {
         const T cd = ucoef(bd + kRyO + d);
-        const int rr = rclamp(row + d, P.op.NY);
-        const int L = AX == 0 ? P.line(rr, P.z, SRC) : P.line(P.y, rr, SRC);
-        T v[W];
+        if (cd != T(0)) {  // uniform: interior D1 rows have a zero centre tap
+            const int rr = rclamp(row + d, P.op.NY);
+            const int L = AX == 0 ? P.line(rr, P.z, SRC) : P.line(P.y, rr, SRC);
+            T v[W];
 #pragma unroll
-        for (int j = 0; j < W; ++j) v[j] = P.ld(L, P.m0 - R + j);
-        EQD_FENCE();  // issue the whole line before the first use
+            for (int j = 0; j < W; ++j) v[j] = P.ld(L, P.m0 - R + j);
+            EQD_FENCE();  // issue the whole line before the first use
 #pragma unroll
-        for (int j = 0; j < W; ++j) a[j] += cd * v[j];
+            for (int j = 0; j < W; ++j) a[j] += cd * v[j];
+        }
     }
 #pragma unroll
     for (int i = 0; i < S; ++i) {
@@ -144,7 +149,8 @@ __device__ __forceinline__ void pen_mixed_x(const PencilCtx<T, S>& P, T (&o)[3][
             const T* b = P.xb.MX + (size_t)(P.m0 + i) * (2 * kRxO + 1);
             T s = T(0);
 #pragma unroll
-            for (int d = -kRxO; d <= kRxO; ++d) s += b[kRxO + d] * a[i + R + d];
+            for (int d = -kRxO; d <= kRxO; ++d)
+                if (i + R + d >= 0 && i + R + d < W) s += b[kRxO + d] * a[i + R + d];
             o[DST][i] += s;
         }
     }
@@ -152,8 +158,8 @@ __device__ __forceinline__ void pen_mixed_x(const PencilCtx<T, S>& P, T (&o)[3][
 
 // x2 (AX = 0) or x3 (AX = 1) second-derivative terms: Ts on the component along AX,
 // To on the other two.
-template <typename T, int S, int AX>
-__device__ __forceinline__ void pen_yz(const PencilCtx<T, S>& P, T (&o)[3][S]) {
+template <typename T, int S, bool F, int AX>
+__device__ __forceinline__ void pen_yz(const PencilCtx<T, S, F>& P, T (&o)[3][S]) {
     constexpr int SC = AX == 0 ? 1 : 2, O1 = 0, O2 = AX == 0 ? 2 : 1;
     EQD_FENCE();
     const int row = AX == 0 ? P.y : P.z;
@@ -163,37 +169,41 @@ __device__ __forceinline__ void pen_yz(const PencilCtx<T, S>& P, T (&o)[3][S]) {
 #pragma unroll EQD_LINE_UNROLL
     for (int d = -kRyS; d <= kRyS; ++d) {  // wide same-direction term
         const T cs = ucoef(bs + kRyS + d);
-        const int L = lines(rclamp(row + d, P.op.NY), SC);
-        T v[S];
+        if (cs != T(0)) {  // uniform: taps past a boundary row's closure are zero
+            const int L = lines(rclamp(row + d, P.op.NY), SC);
+            T v[S];
 #pragma unroll
-        for (int i = 0; i < S; ++i) v[i] = P.ld(L, P.m0 + i);
-        EQD_FENCE();
+            for (int i = 0; i < S; ++i) v[i] = P.ld(L, P.m0 + i);
+            EQD_FENCE();
 #pragma unroll
-        for (int i = 0; i < S; ++i) o[SC][i] += cs * v[i];
+            for (int i = 0; i < S; ++i) o[SC][i] += cs * v[i];
+        }
     }
 #pragma unroll EQD_LINE_UNROLL
     for (int d = -kRyO; d <= kRyO; ++d) {  // narrow term on the other two components
         const T co = ucoef(bo + kRyO + d);
-        const int rr = rclamp(row + d, P.op.NY);
-        const int La = lines(rr, O1), Lb = lines(rr, O2);
-        T va[S], vb[S];
+        if (co != T(0)) {
+            const int rr = rclamp(row + d, P.op.NY);
+            const int La = lines(rr, O1), Lb = lines(rr, O2);
+            T va[S], vb[S];
 #pragma unroll
-        for (int i = 0; i < S; ++i) {
-            va[i] = P.ld(La, P.m0 + i);
-            vb[i] = P.ld(Lb, P.m0 + i);
-        }
-        EQD_FENCE();
+            for (int i = 0; i < S; ++i) {
+                va[i] = P.ld(La, P.m0 + i);
+                vb[i] = P.ld(Lb, P.m0 + i);
+            }
+            EQD_FENCE();
 #pragma unroll
-        for (int i = 0; i < S; ++i) {
-            o[O1][i] += co * va[i];
-            o[O2][i] += co * vb[i];
+            for (int i = 0; i < S; ++i) {
+                o[O1][i] += co * va[i];
+                o[O2][i] += co * vb[i];
+            }
         }
     }
 }
 
 // Mixed x2-x3: (lam+mu) D1y D1z, coupling components 1 and 2.
-template <typename T, int S>
-__device__ __forceinline__ void pen_mixed_yz(const PencilCtx<T, S>& P, T (&o)[3][S]) {
+template <typename T, int S, bool F>
+__device__ __forceinline__ void pen_mixed_yz(const PencilCtx<T, S, F>& P, T (&o)[3][S]) {
     EQD_FENCE();
     const T* by = P.op.D1s + (size_t)P.y * (2 * kRyO + 1);
     const T* bz = P.op.D1 + (size_t)P.z * (2 * kRyO + 1);
@@ -201,47 +211,49 @@ __device__ __forceinline__ void pen_mixed_yz(const PencilCtx<T, S>& P, T (&o)[3]
     for (int t = 0; t < (2 * kRyO + 1) * (2 * kRyO + 1); ++t) {
         const int dy = t / (2 * kRyO + 1) - kRyO, dz = t % (2 * kRyO + 1) - kRyO;
         const T cc = ucoef(by + kRyO + dy) * ucoef(bz + kRyO + dz);
-        const int yy = rclamp(P.y + dy, P.op.NY), zz = rclamp(P.z + dz, P.op.NY);
-        const int L1 = P.line(yy, zz, 1), L2 = P.line(yy, zz, 2);
-        T v1[S], v2[S];
+        if (cc != T(0)) {  // 9 of the 25 combinations vanish on interior rows
+            const int yy = rclamp(P.y + dy, P.op.NY), zz = rclamp(P.z + dz, P.op.NY);
+            const int L1 = P.line(yy, zz, 1), L2 = P.line(yy, zz, 2);
+            T v1[S], v2[S];
 #pragma unroll
-        for (int i = 0; i < S; ++i) {
-            v2[i] = P.ld(L2, P.m0 + i);
-            v1[i] = P.ld(L1, P.m0 + i);
-        }
-        EQD_FENCE();
+            for (int i = 0; i < S; ++i) {
+                v2[i] = P.ld(L2, P.m0 + i);
+                v1[i] = P.ld(L1, P.m0 + i);
+            }
+            EQD_FENCE();
 #pragma unroll
-        for (int i = 0; i < S; ++i) {
-            o[1][i] += cc * v2[i];
-            o[2][i] += cc * v1[i];
+            for (int i = 0; i < S; ++i) {
+                o[1][i] += cc * v2[i];
+                o[2][i] += cc * v1[i];
+            }
         }
     }
 }
 
-template <typename T, int S>
+template <typename T, int S, bool F>
 __device__ __forceinline__ void pencil_apply(const PencilView<T>& op, const XBands<T>& xb, __amdgpu_buffer_rsrc_t rs,
                                              int m0, int y, int z, int lane, T (&o)[3][S]) {
-    const PencilCtx<T, S> P{op, xb, rs, lane * (int)sizeof(T), m0, y, z};
+    const PencilCtx<T, S, F> P{op, xb, rs, lane * (int)sizeof(T), m0, y, z};
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int i = 0; i < S; ++i) o[c][i] = T(0);
-    pen_x<T, S, 0>(P, o);
-    pen_x<T, S, 1>(P, o);
-    pen_x<T, S, 2>(P, o);
-    pen_mixed_x<T, S, 0, 0, 1>(P, o);  // out1 += (lam+mu) D1x D1y u0
-    pen_mixed_x<T, S, 0, 1, 0>(P, o);  // out0 += (lam+mu) D1x D1y u1
-    pen_mixed_x<T, S, 1, 0, 2>(P, o);  // out2 += (lam+mu) D1x D1z u0
-    pen_mixed_x<T, S, 1, 2, 0>(P, o);  // out0 += (lam+mu) D1x D1z u2
-    pen_yz<T, S, 0>(P, o);
-    pen_yz<T, S, 1>(P, o);
-    pen_mixed_yz<T, S>(P, o);
+    pen_x<T, S, F, 0>(P, o);
+    pen_x<T, S, F, 1>(P, o);
+    pen_x<T, S, F, 2>(P, o);
+    pen_mixed_x<T, S, F, 0, 0, 1>(P, o);  // out1 += (lam+mu) D1x D1y u0
+    pen_mixed_x<T, S, F, 0, 1, 0>(P, o);  // out0 += (lam+mu) D1x D1y u1
+    pen_mixed_x<T, S, F, 1, 0, 2>(P, o);  // out2 += (lam+mu) D1x D1z u0
+    pen_mixed_x<T, S, F, 1, 2, 0>(P, o);  // out0 += (lam+mu) D1x D1z u2
+    pen_yz<T, S, F, 0>(P, o);
+    pen_yz<T, S, F, 1>(P, o);
+    pen_mixed_yz<T, S, F>(P, o);
 }
 
 // MODE 0: q = A p with the weighted partial p.q.
 // MODE 1: reliable-update residual r = b - A x (FP64 in, r stored as R for flagged
 //         lanes) with the weighted partial r.r of the FP64 residual.
-template <typename T, typename R, int MODE>
+template <typename T, typename R, int MODE, bool F>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EQD_PENCIL_WAVES))) void k_pencil(
     PencilView<T> op, int NN, const T* __restrict__ in, T* __restrict__ q, const T* __restrict__ b,
     R* __restrict__ rout, const ColState* __restrict__ cs, const int* __restrict__ tile_on,
@@ -272,7 +284,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EQD_PENC
             const int seg = pidx % op.nseg, yz = pidx / op.nseg, y = yz % op.NY, z = yz / op.NY;
             const int m0 = seg * S;
             T o[3][S];
-            pencil_apply<T, S>(op, xb, rs, m0, y, z, lane, o);
+            pencil_apply<T, S, F>(op, xb, rs, m0, y, z, lane, o);
             EQD_FENCE();
             const T hyz = op.Hy[y] * op.Hy[z];
 #pragma unroll
