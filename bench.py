@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--L_normal", type=float, default=400.0)
     ap.add_argument("--l_f", type=float, default=800.0)
     ap.add_argument("--tile_k", type=int, default=64)
+    ap.add_argument("--reliable_interval", type=int, default=0,
+                    help="mixed precision: iterations between FP64 residual refreshes (0 = library default)")
     ap.add_argument("--plain_cols", type=int, default=16)
     ap.add_argument("--cpu_cols", type=int, default=64)
     ap.add_argument("--no_cpu", action="store_true")
@@ -113,7 +115,8 @@ def main():
     mu, nu = 32.04e9, 0.25
     lam = 2 * mu * nu / (1 - 2 * nu)
     g = eqd.bp8_grid_pair(args.dz, args.L_fault, args.L_normal)
-    fe = eqd.FaultElasticity(g, lam, mu, l_f=args.l_f, precision="mixed", tile_k=args.tile_k)
+    fe = eqd.FaultElasticity(g, lam, mu, l_f=args.l_f, precision="mixed", tile_k=args.tile_k,
+                             reliable_interval=args.reliable_interval)
     nf = eqd.frictional_node_count(fe)
     nf2 = 2 * nf
     ncols = nf2
@@ -172,7 +175,8 @@ def main():
         print(f"bench: plain fp64 {len(sample)} cols in {tp:.2f} s ({plain_value:.1f} cols/s), "
               f"vs batched fp64 {err_plain:.2e}", file=sys.stderr, flush=True)
         # stencil kernel timing (events around every launch) for the roofline
-        fet = eqd.FaultElasticity(g, lam, mu, l_f=args.l_f, precision="mixed", tile_k=args.tile_k, timing=1)
+        fet = eqd.FaultElasticity(g, lam, mu, l_f=args.l_f, precision="mixed", tile_k=args.tile_k, timing=1,
+                                  reliable_interval=args.reliable_interval)
         build_device(fet.rs, Kbuf, nf2)
         names = ["stencil", "update", "pupdate", "finalize", "xaccum", "residual64"]
         kt = {n: fet.rs.kernel_timing(i) for i, n in enumerate(names)}

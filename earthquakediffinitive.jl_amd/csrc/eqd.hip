@@ -294,7 +294,7 @@ struct Engine {
         Workspace& w = *s->ws;
         const eqd_op& op = *s->op;
         const bool mixed = std::is_same<T, float>::value;
-        const int R = s->o.reliable_interval > 0 ? s->o.reliable_interval : 16;
+        const int R = s->o.reliable_interval > 0 ? s->o.reliable_interval : 32;
         dim3 grid(kGroups, w.ntiles);
         T* r = w.r.as<T>();
         T* p = w.p.as<T>();
@@ -333,8 +333,8 @@ struct Engine {
             if constexpr (std::is_same<T, float>::value) xu = w.xc.as<float>();
             else xu = w.x.as<double>();
             if constexpr (K == 64)
-                hipLaunchKernelGGL((k_update64<T>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface, xu, r, p,
-                                   q, cs, on, part);
+                hipLaunchKernelGGL((k_update64<T>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface, r, q, cs,
+                                   on, part);
             else
                 hipLaunchKernelGGL((k_update<T, T, K>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface, xu,
                                    r, p, q, cs, on, part);
@@ -344,8 +344,11 @@ struct Engine {
                                nullptr);
         });
         timed(2, [&] {
+            T* xu = nullptr;
+            if constexpr (std::is_same<T, float>::value) xu = w.xc.as<float>();
+            else xu = w.x.as<double>();
             if constexpr (K == 64)
-                hipLaunchKernelGGL((k_pupdate64<T>), grid, dim3(kBlock), 0, st, op.NN, p, r, cs, on);
+                hipLaunchKernelGGL((k_pupdate64<T>), grid, dim3(kBlock), 0, st, op.NN, xu, p, r, cs);
             else
                 hipLaunchKernelGGL((k_pupdate<T, K>), grid, dim3(kBlock), 0, st, op.NN, p, r, cs, on);
         });
@@ -830,7 +833,7 @@ void eqd_cg_opts_default(eqd_cg_opts* o) {
     o->engine = EQD_ENGINE_BATCHED;
     o->tile_k = 0;
     o->max_cols_in_flight = 0;
-    o->reliable_interval = 16;
+    o->reliable_interval = 32;
     o->graph_iters = 8;
 }
 

@@ -12,6 +12,10 @@
 namespace eqd {
 
 constexpr int kGroups = 64;
+#ifndef EQD_CANDIDATE_EVERY
+#define EQD_CANDIDATE_EVERY 4
+#endif
+constexpr int kCandidateEvery = EQD_CANDIDATE_EVERY;
 constexpr int kBlock = 256;
 
 enum ColStatus : int { kActive = 0, kSolved = 1, kTired = 2, kZeroCurv = 3, kIdle = 4 };
@@ -27,6 +31,8 @@ struct ColState {
     int status;
     int since_rel;  // iterations since the last reliable update (mixed precision)
     int rel_flag;   // 1: a reliable update is due after this iteration
+    int xpend;      // 1: x += alpha p of this iteration is still to be applied (64-column tiles)
+    int pad_;
     int64_t itmax;
     int64_t col;    // global column id (-1 = idle slot)
 };
@@ -344,13 +350,13 @@ __device__ __forceinline__ void block_partial_vec(const double (&acc)[VW], doubl
     }
 }
 
-// x += alpha p, r -= alpha q and the weighted partial r.r, for 64-column tiles with
-// 16-byte accesses (the generic k_update moves 4 or 8 bytes per lane).
+// r -= alpha q and the weighted partial r.r, for 64-column tiles with 16-byte accesses.
+// (x += alpha p is applied by k_pupdate64, which reads p anyway: 8 vector passes per
+// iteration instead of 9.)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_update64(int NN, int NX, int mface, T* __restrict__ x, T* __restrict__ r,
-                                                     const T* __restrict__ p, const T* __restrict__ q,
-                                                     const ColState* __restrict__ cs, const int* __restrict__ tile_on,
-                                                     double* __restrict__ partial) {
+__global__ __launch_bounds__(kBlock) void k_update64(int NN, int NX, int mface, T* __restrict__ r,
+                                                     const T* __restrict__ q, const ColState* __restrict__ cs,
+                                                     const int* __restrict__ tile_on, double* __restrict__ partial) {
     using V = typename Vec16<T>::type;
     constexpr int VW = Vec16<T>::n, TPR = 64 / VW;
     int tile, g;
@@ -370,60 +376,72 @@ __global__ __launch_bounds__(kBlock) void k_update64(int NN, int NX, int mface, 
         }
         const size_t off = (size_t)tile * NN * 192;
         const size_t nv = (size_t)NN * 192 / VW;
-        V* xv = reinterpret_cast<V*>(x + off);
         V* rv = reinterpret_cast<V*>(r + off);
-        const V* pv = reinterpret_cast<const V*>(p + off);
         const V* qv = reinterpret_cast<const V*>(q + off);
         for (size_t v = (size_t)g * kBlock + t; v < nv; v += (size_t)kGroups * kBlock) {
             const int node = (int)(v * VW / 192);
             const double w = (node % NX == mface) ? 2.0 : 1.0;
-            V xx = xv[v], rr = rv[v];
-            const V pp = pv[v], qq = qv[v];
+            V rr = rv[v];
+            const V qq = qv[v];
 #pragma unroll
             for (int j = 0; j < VW; ++j) {
                 if (act[j]) {
-                    vset<T>(xx, j, vget<T>(xx, j) + a[j] * vget<T>(pp, j));
                     const T rn = vget<T>(rr, j) - a[j] * vget<T>(qq, j);
                     vset<T>(rr, j, rn);
                     acc[j] += w * ((double)rn * (double)rn);
                 }
             }
-            xv[v] = xx;
             rv[v] = rr;
         }
     }
     block_partial_vec<VW>(acc, partial, tile, g);
 }
 
-// p = r + beta p for 64-column tiles with 16-byte accesses.
+// x += alpha p (columns updated this iteration) and p = r + beta p (columns still
+// active), for 64-column tiles with 16-byte accesses.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pupdate64(int NN, T* __restrict__ p, const T* __restrict__ r,
-                                                      const ColState* __restrict__ cs, const int* __restrict__ tile_on) {
+__global__ __launch_bounds__(kBlock) void k_pupdate64(int NN, T* __restrict__ x, T* __restrict__ p,
+                                                      const T* __restrict__ r, const ColState* __restrict__ cs) {
     using V = typename Vec16<T>::type;
     constexpr int VW = Vec16<T>::n, TPR = 64 / VW;
     int tile, g;
     tile_group(tile, g);
-    if (!tile_on[tile]) return;
     const int t = threadIdx.x, c0 = (t % TPR) * VW;
-    T b[VW];
-    bool act[VW];
+    T a[VW], b[VW];
+    bool xu[VW], pu[VW];
+    bool anyx = false, anyp = false;
 #pragma unroll
     for (int j = 0; j < VW; ++j) {
         const ColState& st = cs[(size_t)tile * 64 + c0 + j];
-        act[j] = st.status == kActive;
+        xu[j] = st.xpend != 0;
+        pu[j] = st.status == kActive;
+        a[j] = (T)st.alpha;
         b[j] = (T)st.beta;
+        anyx |= xu[j];
+        anyp |= pu[j];
     }
+    if (!anyx && !anyp) return;
     const size_t off = (size_t)tile * NN * 192;
     const size_t nv = (size_t)NN * 192 / VW;
+    V* xv = reinterpret_cast<V*>(x + off);
     V* pv = reinterpret_cast<V*>(p + off);
     const V* rv = reinterpret_cast<const V*>(r + off);
     for (size_t v = (size_t)g * kBlock + t; v < nv; v += (size_t)kGroups * kBlock) {
         V pp = pv[v];
-        const V rr = rv[v];
+        if (anyx) {
+            V xx = xv[v];
 #pragma unroll
-        for (int j = 0; j < VW; ++j)
-            if (act[j]) vset<T>(pp, j, vget<T>(rr, j) + b[j] * vget<T>(pp, j));
-        pv[v] = pp;
+            for (int j = 0; j < VW; ++j)
+                if (xu[j]) vset<T>(xx, j, vget<T>(xx, j) + a[j] * vget<T>(pp, j));
+            xv[v] = xx;
+        }
+        if (anyp) {
+            const V rr = rv[v];
+#pragma unroll
+            for (int j = 0; j < VW; ++j)
+                if (pu[j]) vset<T>(pp, j, vget<T>(rr, j) + b[j] * vget<T>(pp, j));
+            pv[v] = pp;
+        }
     }
 }
 
@@ -469,6 +487,7 @@ __global__ void k_finalize(int mode, int mixed, int rel_interval, ColState* __re
     const int tile = blockIdx.x, lane = threadIdx.x;
     ColState& st = cs[(size_t)tile * K + lane];
     const double meps = 2.220446049250313e-16;
+    if (mode == 0) st.xpend = 0;  // the previous iteration's p update has run
     if (st.status == kActive) {
         const double s = sum_partials<K>(partial, tile, lane);
         if (mode == 0) {
@@ -478,6 +497,7 @@ __global__ void k_finalize(int mode, int mixed, int rel_interval, ColState* __re
                 st.alpha = st.rr / s;
             }
         } else if (mode == 1) {
+            st.xpend = 1;  // updated this iteration, whatever the convergence test says
             const double rnorm = sqrt(s);
             const bool solved = rnorm <= st.eps || rnorm + 1.0 <= 1.0;
             st.iters += 1;
@@ -490,7 +510,12 @@ __global__ void k_finalize(int mode, int mixed, int rel_interval, ColState* __re
                 st.beta = beta;
                 st.pnorm2 = s + beta * beta * st.pnorm2;
                 st.rr = s;
-                if (mixed && (solved || st.since_rel >= rel_interval)) st.rel_flag = 1;
+                // A column whose FP32 residual claims convergence is checked in FP64 only on
+                // iterations divisible by kCandidateEvery: the columns of a tile then share
+                // their checks instead of each triggering a full-tile FP64 pass. The schedule
+                // depends on the column's own iteration count only.
+                if (mixed && ((solved && st.iters % kCandidateEvery == 0) || st.since_rel >= rel_interval))
+                    st.rel_flag = 1;
                 if (st.iters >= st.itmax) {
                     st.status = kTired;
                     st.rel_flag = mixed ? 1 : 0;  // fold the last correction into x
@@ -609,6 +634,8 @@ __global__ void k_init_state(ColState* __restrict__ cs, const double* __restrict
     st.iters = 0;
     st.since_rel = 0;
     st.rel_flag = 0;
+    st.xpend = 0;
+    st.pad_ = 0;
     st.itmax = itmax;
     if (st.col < 0) st.status = kIdle;
     else if (s == 0.0 || st.bnorm <= st.eps) st.status = kSolved;  // x = 0 is the answer
