@@ -80,6 +80,15 @@ struct PencilCtx {
     __device__ T ld(int lineoff, int mm) const { return bload<T>(rs, voff, lineoff + xc(mm) * C); }
 };
 
+// Wave-uniform read of a band coefficient through the constant address space, so it is
+// a scalar load (counted on lgkmcnt): a vector load here would share vmcnt with the
+// data loads and every wait for a coefficient would drain all loads in flight.
+template <typename T>
+__device__ __forceinline__ T ucoef(const T* p) {
+    using CP = const __attribute__((address_space(4))) T*;
+    return *(CP)p;
+}
+
 // x1 term of component Cc: AXs for component 0, AXo for 1 and 2.
 template <typename T, int S, bool F, int Cc>
 __device__ __forceinline__ void pen_x(const PencilCtx<T, S, F>& P, T (&o)[3][S]) {
@@ -91,13 +100,20 @@ __device__ __forceinline__ void pen_x(const PencilCtx<T, S, F>& P, T (&o)[3][S])
 #pragma unroll
     for (int j = 0; j < W; ++j) w[j] = P.ld(L, P.m0 - R + j);
     const T* band = Cc == 0 ? P.xb.AXs : P.xb.AXo;
+    // The centre taps of the x2 and x3 terms act on this same line (the pencil's own
+    // nodes): they are applied here instead of loading the line again.
+    const T cy = Cc == 1 ? ucoef(P.op.Ts + (size_t)P.y * (2 * kRyS + 1) + kRyS)
+                         : ucoef(P.op.To + (size_t)P.y * (2 * kRyO + 1) + kRyO);
+    const T cz = Cc == 2 ? ucoef(P.op.Ts + (size_t)P.z * (2 * kRyS + 1) + kRyS)
+                         : ucoef(P.op.To + (size_t)P.z * (2 * kRyO + 1) + kRyO);
+    const T cyz = cy + cz;
 #pragma unroll
     for (int i = 0; i < S; ++i) {
         // A real (uniform) branch per row: keeps the compiler from hoisting every row's
         // coefficient reads to the top of the block (which needs hundreds of registers).
         if (P.m0 + i < P.op.NX) {
             const T* b = band + (size_t)(P.m0 + i) * (2 * RB + 1);
-            T a = T(0);
+            T a = cyz * w[i + R];
 #pragma unroll
             for (int d = -RB; d <= RB; ++d)
                 if (i + R + d >= 0 && i + R + d < W) a += b[RB + d] * w[i + R + d];
@@ -106,20 +122,12 @@ __device__ __forceinline__ void pen_x(const PencilCtx<T, S, F>& P, T (&o)[3][S])
     }
 }
 
-// Wave-uniform read of a band coefficient through the constant address space, so it is
-// a scalar load (counted on lgkmcnt): a vector load here would share vmcnt with the
-// data loads and every wait for a coefficient would drain all loads in flight.
-template <typename T>
-__device__ __forceinline__ T ucoef(const T* p) {
-    using CP = const __attribute__((address_space(4))) T*;
-    return *(CP)p;
-}
-
 // Neighbour rows outside [0, NY) are clamped to a valid row: their band coefficients
 // are zero (the tables only couple interior rows), so the loads stay branch-free.
 __device__ __forceinline__ int rclamp(int r, int n) { return r < 0 ? 0 : (r >= n ? n - 1 : r); }
 
-// Mixed x1-x2 (AX = 0) or x1-x3 (AX = 1): o[DST] += MX (D1 along AX of component SRC).
+// Mixed x1-x2 (AX = 0) or x1-x3 (AX = 1) as a pass of its own: o[DST] += MX (D1 along AX of
+// component SRC). Used when the pencil has x1 halos (the fused form would spill there).
 template <typename T, int S, bool F, int AX, int SRC, int DST>
 __device__ __forceinline__ void pen_mixed_x(const PencilCtx<T, S, F>& P, T (&o)[3][S]) {
     constexpr int R = F ? 0 : kRxO, W = S + 2 * R;
@@ -156,21 +164,60 @@ __device__ __forceinline__ void pen_mixed_x(const PencilCtx<T, S, F>& P, T (&o)[
     }
 }
 
-// x2 (AX = 0) or x3 (AX = 1) second-derivative terms: Ts on the component along AX,
-// To on the other two.
+// (lam+mu) D1x applied to an accumulated x1 window: o[DST][i] += sum_dx MX[m][dx] a[i + R + dx].
+template <typename T, int S, bool F, int DST, int W>
+__device__ __forceinline__ void apply_mx(const PencilCtx<T, S, F>& P, const T (&a)[W], T (&o)[3][S]) {
+    constexpr int R = (W - S) / 2;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        if (P.m0 + i < P.op.NX) {  // per-row branch, as in pen_x
+            const T* b = P.xb.MX + (size_t)(P.m0 + i) * (2 * kRxO + 1);
+            T s = T(0);
+#pragma unroll
+            for (int d = -kRxO; d <= kRxO; ++d)
+                if (i + R + d >= 0 && i + R + d < W) s += b[kRxO + d] * a[i + R + d];
+            o[DST][i] += s;
+        }
+    }
+}
+
+// x2 (AX = 0) or x3 (AX = 1) terms: the second-derivative bands (Ts on the component along
+// AX, To on the other two) and, from the same lines, the mixed x1-x2 / x1-x3 terms
+// (lam+mu) D1x D1_AX: the D1_AX sum of a source component is accumulated over its lines
+// and D1x is applied once at the end. Source components: the component along AX feeds
+// o[0]; component 0 feeds o[AX == 0 ? 1 : 2]. The centre taps (d = 0) are in pen_x.
+// The mixed terms are fused only for full-line pencils (F); otherwise pen_mixed_x does them.
 template <typename T, int S, bool F, int AX>
 __device__ __forceinline__ void pen_yz(const PencilCtx<T, S, F>& P, T (&o)[3][S]) {
     constexpr int SC = AX == 0 ? 1 : 2, O1 = 0, O2 = AX == 0 ? 2 : 1;
+    constexpr int DA = 0, DB = AX == 0 ? 1 : 2;
+    constexpr int RM = 0, WM = S;  // x1 window of the mixed term (full-line pencils need no halo)
     EQD_FENCE();
     const int row = AX == 0 ? P.y : P.z;
     const T* bs = P.op.Ts + (size_t)row * (2 * kRyS + 1);
     const T* bo = P.op.To + (size_t)row * (2 * kRyO + 1);
+    const T* bd = P.op.D1 + (size_t)row * (2 * kRyO + 1);
     auto lines = [&](int rr, int c) { return AX == 0 ? P.line(rr, P.z, c) : P.line(P.y, rr, c); };
+    T a[WM];
+    // component SC: Ts, and D1 into the mixed accumulator for o[DA]
+#pragma unroll
+    for (int j = 0; j < WM; ++j) a[j] = T(0);
 #pragma unroll EQD_LINE_UNROLL
-    for (int d = -kRyS; d <= kRyS; ++d) {  // wide same-direction term
+    for (int d = -kRyS; d <= kRyS; ++d) {
+        if (d == 0) continue;
         const T cs = ucoef(bs + kRyS + d);
-        if (cs != T(0)) {  // uniform: taps past a boundary row's closure are zero
-            const int L = lines(rclamp(row + d, P.op.NY), SC);
+        const T cd = (F && d >= -kRyO && d <= kRyO) ? ucoef(bd + kRyO + d) : T(0);
+        const int L = lines(rclamp(row + d, P.op.NY), SC);
+        if (cd != T(0)) {  // uniform branches: skip what vanishes on this row
+            T v[WM];
+#pragma unroll
+            for (int j = 0; j < WM; ++j) v[j] = P.ld(L, P.m0 - RM + j);
+            EQD_FENCE();
+#pragma unroll
+            for (int i = 0; i < S; ++i) o[SC][i] += cs * v[i + RM];
+#pragma unroll
+            for (int j = 0; j < WM; ++j) a[j] += cd * v[j];
+        } else if (cs != T(0)) {
             T v[S];
 #pragma unroll
             for (int i = 0; i < S; ++i) v[i] = P.ld(L, P.m0 + i);
@@ -179,26 +226,37 @@ __device__ __forceinline__ void pen_yz(const PencilCtx<T, S, F>& P, T (&o)[3][S]
             for (int i = 0; i < S; ++i) o[SC][i] += cs * v[i];
         }
     }
+    if constexpr (F) apply_mx<T, S, F, DA, WM>(P, a, o);
+    EQD_FENCE();
+    // components O1 (= 0) and O2: To, and D1 of component 0 into the accumulator for o[DB]
+#pragma unroll
+    for (int j = 0; j < WM; ++j) a[j] = T(0);
 #pragma unroll EQD_LINE_UNROLL
-    for (int d = -kRyO; d <= kRyO; ++d) {  // narrow term on the other two components
+    for (int d = -kRyO; d <= kRyO; ++d) {
+        if (d == 0) continue;
         const T co = ucoef(bo + kRyO + d);
-        if (co != T(0)) {
+        const T cd = F ? ucoef(bd + kRyO + d) : T(0);
+        if (co != T(0) || cd != T(0)) {
             const int rr = rclamp(row + d, P.op.NY);
             const int La = lines(rr, O1), Lb = lines(rr, O2);
-            T va[S], vb[S];
+            T va[WM], vb[S];
 #pragma unroll
-            for (int i = 0; i < S; ++i) {
-                va[i] = P.ld(La, P.m0 + i);
-                vb[i] = P.ld(Lb, P.m0 + i);
-            }
+            for (int j = 0; j < WM; ++j) va[j] = P.ld(La, P.m0 - RM + j);
+#pragma unroll
+            for (int i = 0; i < S; ++i) vb[i] = P.ld(Lb, P.m0 + i);
             EQD_FENCE();
 #pragma unroll
             for (int i = 0; i < S; ++i) {
-                o[O1][i] += co * va[i];
+                o[O1][i] += co * va[i + RM];
                 o[O2][i] += co * vb[i];
+            }
+            if constexpr (F) {
+#pragma unroll
+                for (int j = 0; j < WM; ++j) a[j] += cd * va[j];
             }
         }
     }
+    if constexpr (F) apply_mx<T, S, F, DB, WM>(P, a, o);
 }
 
 // Mixed x2-x3: (lam+mu) D1y D1z, coupling components 1 and 2.
@@ -241,12 +299,14 @@ __device__ __forceinline__ void pencil_apply(const PencilView<T>& op, const XBan
     pen_x<T, S, F, 0>(P, o);
     pen_x<T, S, F, 1>(P, o);
     pen_x<T, S, F, 2>(P, o);
-    pen_mixed_x<T, S, F, 0, 0, 1>(P, o);  // out1 += (lam+mu) D1x D1y u0
-    pen_mixed_x<T, S, F, 0, 1, 0>(P, o);  // out0 += (lam+mu) D1x D1y u1
-    pen_mixed_x<T, S, F, 1, 0, 2>(P, o);  // out2 += (lam+mu) D1x D1z u0
-    pen_mixed_x<T, S, F, 1, 2, 0>(P, o);  // out0 += (lam+mu) D1x D1z u2
-    pen_yz<T, S, F, 0>(P, o);
-    pen_yz<T, S, F, 1>(P, o);
+    if constexpr (!F) {
+        pen_mixed_x<T, S, F, 0, 0, 1>(P, o);  // out1 += (lam+mu) D1x D1y u0
+        pen_mixed_x<T, S, F, 0, 1, 0>(P, o);  // out0 += (lam+mu) D1x D1y u1
+        pen_mixed_x<T, S, F, 1, 0, 2>(P, o);  // out2 += (lam+mu) D1x D1z u0
+        pen_mixed_x<T, S, F, 1, 2, 0>(P, o);  // out0 += (lam+mu) D1x D1z u2
+    }
+    pen_yz<T, S, F, 0>(P, o);  // F: + out0 += (lam+mu) D1x D1y u1, out1 += (lam+mu) D1x D1y u0
+    pen_yz<T, S, F, 1>(P, o);  // F: + out0 += (lam+mu) D1x D1z u2, out2 += (lam+mu) D1x D1z u0
     pen_mixed_yz<T, S, F>(P, o);
 }
 
