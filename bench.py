@@ -189,7 +189,9 @@ def main():
         # whole-iteration metric of the survey: 10 vector passes x DOF x 4 B per column-iteration
         col_iters = rep["iterations"] / max(1, rep["solves"]) * ncols
         survey_gbs = 10 * dof * 4 * col_iters / (ms_per_step / 1e3)
-        stencil_name = ("k_pencil<float,float,0>" if args.tile_k == 64
+        # full-line pencils (one 16-node segment covers x1: 2 n1 - 3 <= 16)
+        full = "true" if 2 * g.n1 - 3 <= 16 else "false"
+        stencil_name = (f"k_pencil<float,float,0,{full}>" if args.tile_k == 64
                         else f"k_apply_dot<float,{args.tile_k}>")
         out["roofline"] = {"bound": "hbm", "kernel": stencil_name,
                            "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
