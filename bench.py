@@ -9,7 +9,9 @@ Besides the timed steps it measures, on the same GPU: the plain one-RHS FP64 CG
 (host alpha/beta, no graphs) on a column sample, the batched FP64 build, K's error
 against the batched FP64 K and against the CPU oracle on sampled columns, the
 stencil kernel's average launch duration (HIP events on the solver's stream) for the
-roofline, and the CPU oracle's threaded build on a column sample.
+roofline, the CPU oracle's threaded build on a column sample, and the BP8 time loop
+(30 days of BP8-QD-GS at dz = 50 m with a device-built Toeplitz K) beside the CPU oracle's
+integration of the same K.
 """
 import argparse
 import ctypes
@@ -84,6 +86,46 @@ def measured_traffic(kernel):
     return None
 
 
+def bench_bp8(args):
+    """BP8-QD-GS for 30 days at dz = 50 m, L = (800, 400), l_f = 400 (run_bp8.jl's defaults:
+    Toeplitz K, saveat 300 s) with K built and kept on the device; the CPU oracle integrates the
+    same K beside it."""
+    from eqd import bp8
+    t0 = time.perf_counter()
+    m = bp8.build_model(dz=50.0, L_fault=800.0, L_normal=400.0, stiffness="toeplitz", precision="mixed")
+    t_build = time.perf_counter() - t0
+    tf, saveat = 30 * 86400.0, 300.0
+    t0 = time.perf_counter()
+    sol = bp8.run_bp8(m, tspan=(0.0, tf), saveat=saveat)
+    t_run = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    vmax = bp8.saved_vmax(m, len(sol.t))
+    t_vmax = time.perf_counter() - t0
+    nf = m.nf
+    ic = (m.n2f // 2) * (m.n2f + 1)
+    j = int(np.argmax(vmax))
+    res = {"nf": nf, "state_length": m.state_len, "K_build_ms": round(t_build * 1e3, 1),
+           "run_ms": round(t_run * 1e3, 1), "device_loop_ms": round(sol.ms, 1), "accepted_steps": sol.naccept,
+           "rejected_steps": sol.nreject, "rhs_evaluations": sol.nrhs, "saved": len(sol.t),
+           "us_per_rhs": round(sol.ms * 1e3 / max(1, sol.nrhs), 2), "saved_vmax_ms": round(t_vmax * 1e3, 1),
+           "peak_V": float(vmax[j]), "peak_V_day": round(float(sol.t[j]) / 86400, 3),
+           "final_slip_centre": float(sol.u[-1, ic]), "peak_p_centre_MPa": float(sol.u[:, 3 * nf + ic].max() / 1e6)}
+    print(f"bench: bp8 30 d: K build {t_build * 1e3:.0f} ms, run {t_run * 1e3:.0f} ms "
+          f"({sol.naccept} steps, {sol.nrhs} rhs), peak V {vmax[j]:.3e}", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        from oracle.oracle import OracleBP8
+        o = OracleBP8(m.K, m.n2f, dict(dz=50.0))
+        t0 = time.perf_counter()
+        ro = o.run(0.0, tf, saveat)
+        t_cpu = time.perf_counter() - t0
+        res["cpu_oracle_run_ms"] = round(t_cpu * 1e3, 1)
+        res["cpu_oracle_accepted_steps"] = ro["naccept"]
+        res["vmax_max_rel_diff_vs_oracle"] = float(np.max(np.abs(vmax - ro["vmax"]) / ro["vmax"]))
+        print(f"bench: bp8 30 d cpu oracle {t_cpu * 1e3:.0f} ms ({ro['naccept']} steps)", file=sys.stderr,
+              flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +142,7 @@ def main():
     ap.add_argument("--cpu_cols", type=int, default=64)
     ap.add_argument("--no_cpu", action="store_true")
     ap.add_argument("--no_extras", action="store_true", help="only the timed mixed-precision steps")
+    ap.add_argument("--no_bp8", action="store_true", help="skip the BP8 time-loop measurement")
     ap.add_argument("--copy_calib_mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (3 times) and exit: the "
                          "reference kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -223,6 +266,8 @@ def main():
             print(f"bench: cpu oracle {len(csample)} cols in {tc:.1f} s on {ncpu} threads; "
                   f"mixed K vs oracle {err_oracle:.2e}", file=sys.stderr, flush=True)
     Kbuf.free()
+    if not args.no_extras and not args.no_bp8:
+        out["bp8_30day_gs"] = bench_bp8(args)
     print(json.dumps(out), flush=True)
 
 
