@@ -186,12 +186,12 @@ __device__ __forceinline__ void apply_mx(const PencilCtx<T, S, F>& P, const T (&
 // (lam+mu) D1x D1_AX: the D1_AX sum of a source component is accumulated over its lines
 // and D1x is applied once at the end. Source components: the component along AX feeds
 // o[0]; component 0 feeds o[AX == 0 ? 1 : 2]. The centre taps (d = 0) are in pen_x.
-// The mixed terms are fused only for full-line pencils (F); otherwise pen_mixed_x does them.
+// Lines that feed a mixed term are loaded with the x1 halo of D1x when the pencil has one.
 template <typename T, int S, bool F, int AX>
 __device__ __forceinline__ void pen_yz(const PencilCtx<T, S, F>& P, T (&o)[3][S]) {
     constexpr int SC = AX == 0 ? 1 : 2, O1 = 0, O2 = AX == 0 ? 2 : 1;
     constexpr int DA = 0, DB = AX == 0 ? 1 : 2;
-    constexpr int RM = 0, WM = S;  // x1 window of the mixed term (full-line pencils need no halo)
+    constexpr int RM = F ? 0 : kRxO, WM = S + 2 * RM;  // x1 window of the mixed term
     EQD_FENCE();
     const int row = AX == 0 ? P.y : P.z;
     const T* bs = P.op.Ts + (size_t)row * (2 * kRyS + 1);
@@ -206,7 +206,7 @@ __device__ __forceinline__ void pen_yz(const PencilCtx<T, S, F>& P, T (&o)[3][S]
     for (int d = -kRyS; d <= kRyS; ++d) {
         if (d == 0) continue;
         const T cs = ucoef(bs + kRyS + d);
-        const T cd = (F && d >= -kRyO && d <= kRyO) ? ucoef(bd + kRyO + d) : T(0);
+        const T cd = (d >= -kRyO && d <= kRyO) ? ucoef(bd + kRyO + d) : T(0);
         const int L = lines(rclamp(row + d, P.op.NY), SC);
         if (cd != T(0)) {  // uniform branches: skip what vanishes on this row
             T v[WM];
@@ -226,7 +226,7 @@ __device__ __forceinline__ void pen_yz(const PencilCtx<T, S, F>& P, T (&o)[3][S]
             for (int i = 0; i < S; ++i) o[SC][i] += cs * v[i];
         }
     }
-    if constexpr (F) apply_mx<T, S, F, DA, WM>(P, a, o);
+    apply_mx<T, S, F, DA, WM>(P, a, o);
     EQD_FENCE();
     // components O1 (= 0) and O2: To, and D1 of component 0 into the accumulator for o[DB]
 #pragma unroll
@@ -235,7 +235,7 @@ __device__ __forceinline__ void pen_yz(const PencilCtx<T, S, F>& P, T (&o)[3][S]
     for (int d = -kRyO; d <= kRyO; ++d) {
         if (d == 0) continue;
         const T co = ucoef(bo + kRyO + d);
-        const T cd = F ? ucoef(bd + kRyO + d) : T(0);
+        const T cd = ucoef(bd + kRyO + d);
         if (co != T(0) || cd != T(0)) {
             const int rr = rclamp(row + d, P.op.NY);
             const int La = lines(rr, O1), Lb = lines(rr, O2);
@@ -250,13 +250,11 @@ __device__ __forceinline__ void pen_yz(const PencilCtx<T, S, F>& P, T (&o)[3][S]
                 o[O1][i] += co * va[i + RM];
                 o[O2][i] += co * vb[i];
             }
-            if constexpr (F) {
 #pragma unroll
-                for (int j = 0; j < WM; ++j) a[j] += cd * va[j];
-            }
+            for (int j = 0; j < WM; ++j) a[j] += cd * va[j];
         }
     }
-    if constexpr (F) apply_mx<T, S, F, DB, WM>(P, a, o);
+    apply_mx<T, S, F, DB, WM>(P, a, o);
 }
 
 // Mixed x2-x3: (lam+mu) D1y D1z, coupling components 1 and 2.
@@ -299,14 +297,8 @@ __device__ __forceinline__ void pencil_apply(const PencilView<T>& op, const XBan
     pen_x<T, S, F, 0>(P, o);
     pen_x<T, S, F, 1>(P, o);
     pen_x<T, S, F, 2>(P, o);
-    if constexpr (!F) {
-        pen_mixed_x<T, S, F, 0, 0, 1>(P, o);  // out1 += (lam+mu) D1x D1y u0
-        pen_mixed_x<T, S, F, 0, 1, 0>(P, o);  // out0 += (lam+mu) D1x D1y u1
-        pen_mixed_x<T, S, F, 1, 0, 2>(P, o);  // out2 += (lam+mu) D1x D1z u0
-        pen_mixed_x<T, S, F, 1, 2, 0>(P, o);  // out0 += (lam+mu) D1x D1z u2
-    }
-    pen_yz<T, S, F, 0>(P, o);  // F: + out0 += (lam+mu) D1x D1y u1, out1 += (lam+mu) D1x D1y u0
-    pen_yz<T, S, F, 1>(P, o);  // F: + out0 += (lam+mu) D1x D1z u2, out2 += (lam+mu) D1x D1z u0
+    pen_yz<T, S, F, 0>(P, o);  // + out0 += (lam+mu) D1x D1y u1, out1 += (lam+mu) D1x D1y u0
+    pen_yz<T, S, F, 1>(P, o);  // + out0 += (lam+mu) D1x D1z u2, out2 += (lam+mu) D1x D1z u0
     pen_mixed_yz<T, S, F>(P, o);
 }
 
@@ -314,7 +306,7 @@ __device__ __forceinline__ void pencil_apply(const PencilView<T>& op, const XBan
 // MODE 1: reliable-update residual r = b - A x (FP64 in, r stored as R for flagged
 //         lanes) with the weighted partial r.r of the FP64 residual.
 template <typename T, typename R, int MODE, bool F>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EQD_PENCIL_WAVES))) void k_pencil(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(F ? EQD_PENCIL_WAVES : EQD_PENCIL_WAVES - 1))) void k_pencil(
     PencilView<T> op, int NN, const T* __restrict__ in, T* __restrict__ q, const T* __restrict__ b,
     R* __restrict__ rout, const ColState* __restrict__ cs, const int* __restrict__ tile_on,
     double* __restrict__ partial) {
