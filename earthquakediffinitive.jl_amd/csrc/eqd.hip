@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -355,7 +357,7 @@ struct Engine {
         if (mixed) {
             timed(4, [&] {
                 hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, op.NN, w.x.as<double>(), w.xc.as<float>(),
-                                   cs, rel, 0);
+                                   cs, rel, 0, (const int*)nullptr);
             });
             timed(5, [&] {
                 if constexpr (K == 64)
@@ -384,7 +386,7 @@ struct Engine {
     }
 
     // Initial state once b, x, r, p (and xc) hold the start of every column.
-    static void init_state(eqd_solver* s, hipStream_t st) {
+    static void init_state(eqd_solver* s, hipStream_t st, const int* mask = nullptr) {
         Workspace& w = *s->ws;
         const eqd_op& op = *s->op;
         dim3 grid(kGroups, w.ntiles);
@@ -393,7 +395,7 @@ struct Engine {
         const int64_t itmax = s->o.itmax > 0 ? s->o.itmax : 2 * op.ntot;
         hipLaunchKernelGGL((k_init_state<K>), dim3(w.ntiles), dim3(K), 0, st, w.cs.as<ColState>(),
                            w.partial.as<double>(), w.colid.as<int64_t>(), s->o.rtol, s->o.atol, itmax,
-                           w.tile_on.as<int>());
+                           w.tile_on.as<int>(), mask);
         HIPCHK(hipGetLastError());
     }
 
@@ -423,7 +425,7 @@ struct Engine {
         if (std::is_same<T, float>::value) {
             dim3 grid(kGroups, w.ntiles);
             hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, s->op->NN, w.x.as<double>(),
-                               w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1);
+                               w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1, (const int*)nullptr);
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipStreamSynchronize(st));
@@ -432,9 +434,8 @@ struct Engine {
         s->last_tile_iters = tile_iters;
     }
 
-    // Iterate until no tile has an active column; graphs of `gi` iterations.
-    static void solve(eqd_solver* s) {
-        if (s->o.timing) return solve_timed(s);
+    // The HIP graph of `gi` CG iterations over the workspace (captured once per shape).
+    static int ensure_graph(eqd_solver* s) {
         Workspace& w = *s->ws;
         hipStream_t st = s->stream;
         const int gi = s->o.graph_iters > 0 ? s->o.graph_iters : 8;
@@ -449,6 +450,21 @@ struct Engine {
             HIPCHK(hipGraphDestroy(graph));
             w.exec_iters = gi;
         }
+        return gi;
+    }
+
+    // One replay of the iteration graph (the serving loop's step).
+    static void replay(eqd_solver* s) {
+        ensure_graph(s);
+        HIPCHK(hipGraphLaunch(s->ws->exec, s->stream));
+    }
+
+    // Iterate until no tile has an active column; graphs of `gi` iterations.
+    static void solve(eqd_solver* s) {
+        if (s->o.timing) return solve_timed(s);
+        Workspace& w = *s->ws;
+        hipStream_t st = s->stream;
+        const int gi = ensure_graph(s);
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
@@ -469,7 +485,7 @@ struct Engine {
         if (std::is_same<T, float>::value) {  // fold any correction still held in FP32
             dim3 grid(kGroups, w.ntiles);
             hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, s->op->NN, w.x.as<double>(),
-                               w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1);
+                               w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1, (const int*)nullptr);
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(e1, st));
@@ -557,23 +573,34 @@ struct Engine {
     }
 
     // Builds b (and the Krylov start vectors) from the slip fields already on the device.
-    static void rhs_from_fields(eqd_solver* s) {
+    static void rhs_from_fields(eqd_solver* s, const int* mask = nullptr) {
         Workspace& w = *s->ws;
         const eqd_op& op = *s->op;
         dim3 grid(kGroups, w.ntiles);
         hipLaunchKernelGGL((k_rhs<T, K>), grid, dim3(kBlock), 0, s->stream, op.rv, op.NN, w.fields.as<double>(),
                            w.ntiles * K, w.b.as<double>(), w.x.as<double>(), w.r.as<T>(), w.p.as<T>(),
-                           std::is_same<T, float>::value ? w.xc.as<float>() : nullptr);
+                           std::is_same<T, float>::value ? w.xc.as<float>() : nullptr, mask);
         HIPCHK(hipGetLastError());
     }
 
-    static void extract(eqd_solver* s, double* Kdev, int64_t ldK) {
+    // Folds the FP32 correction of the masked columns into x (mixed precision).
+    static void fold(eqd_solver* s, const int* mask) {
+        if constexpr (std::is_same<T, float>::value) {
+            Workspace& w = *s->ws;
+            dim3 grid(kGroups, w.ntiles);
+            hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, s->stream, s->op->NN, w.x.as<double>(),
+                               w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1, mask);
+            HIPCHK(hipGetLastError());
+        }
+    }
+
+    static void extract(eqd_solver* s, double* Kdev, int64_t ldK, const int* mask = nullptr) {
         Workspace& w = *s->ws;
         const eqd_op& op = *s->op;
         const int npb = kBlock / K;
         dim3 grid((op.nf + npb - 1) / npb, w.ntiles);
         hipLaunchKernelGGL((k_extract<K>), grid, dim3(kBlock), 0, s->stream, op.ev, op.NN, w.x.as<double>(),
-                           w.fields.as<double>(), w.outpos.as<int64_t>(), Kdev, ldK, (int64_t)0);
+                           w.fields.as<double>(), w.outpos.as<int64_t>(), Kdev, ldK, (int64_t)0, mask);
         HIPCHK(hipGetLastError());
     }
 };
@@ -587,15 +614,24 @@ struct Dispatch {
         } else if (s->ws->prec == EQD_PREC_MIXED) Engine<float, K>::solve(s);
         else Engine<double, K>::solve(s);
     }
-    static void init(eqd_solver* s) {
-        if (s->ws->prec == EQD_PREC_MIXED) Engine<float, K>::init_state(s, s->stream);
-        else Engine<double, K>::init_state(s, s->stream);
+    static void init(eqd_solver* s, const int* mask = nullptr) {
+        if (s->ws->prec == EQD_PREC_MIXED) Engine<float, K>::init_state(s, s->stream, mask);
+        else Engine<double, K>::init_state(s, s->stream, mask);
     }
-    static void rhs(eqd_solver* s) {
-        if (s->ws->prec == EQD_PREC_MIXED) Engine<float, K>::rhs_from_fields(s);
-        else Engine<double, K>::rhs_from_fields(s);
+    static void rhs(eqd_solver* s, const int* mask = nullptr) {
+        if (s->ws->prec == EQD_PREC_MIXED) Engine<float, K>::rhs_from_fields(s, mask);
+        else Engine<double, K>::rhs_from_fields(s, mask);
     }
-    static void extract(eqd_solver* s, double* Kdev, int64_t ldK) { Engine<double, K>::extract(s, Kdev, ldK); }
+    static void fold(eqd_solver* s, const int* mask) {
+        if (s->ws->prec == EQD_PREC_MIXED) Engine<float, K>::fold(s, mask);
+    }
+    static void replay(eqd_solver* s) {
+        if (s->ws->prec == EQD_PREC_MIXED) Engine<float, K>::replay(s);
+        else Engine<double, K>::replay(s);
+    }
+    static void extract(eqd_solver* s, double* Kdev, int64_t ldK, const int* mask = nullptr) {
+        Engine<double, K>::extract(s, Kdev, ldK, mask);
+    }
 };
 
 template <typename F>
@@ -687,7 +723,7 @@ void run_columns(eqd_solver* s, const std::vector<int64_t>& colid, const std::ve
 void unit_fill(eqd_solver* s, Workspace& w, int slots) {
     const eqd_op& op = *s->op;
     hipLaunchKernelGGL(k_unit_fields, dim3(slots), dim3(256), 0, s->stream, op.g.n23, op.i2lo, op.n2f, op.nf, slots,
-                       w.colid.as<int64_t>(), w.fields.as<double>());
+                       w.colid.as<int64_t>(), w.fields.as<double>(), (const int*)nullptr);
     HIPCHK(hipGetLastError());
 }
 
@@ -782,6 +818,167 @@ void stiffness(eqd_solver* s, int mode, const int* cols, int ncols, double* Kdev
                        dker.as<double>(), dhave.as<int>(), Kdev, ldK);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+// Serving loop: continuous batching of stiffness-column and forward-solve requests.
+//
+// Each grid added to a server owns an operator, a solver and a resident pool of 64-column
+// tiles sized by the server's HBM budget. A step retires the columns that finished (their
+// FP32 correction folded into x, tractions extracted into a per-slot result arena and
+// copied to the request), admits queued requests into the freed lanes (slip fields, RHS and
+// Krylov start built for those lanes only), and replays the CG iteration graph once. A
+// column's arithmetic depends on its own lane only, so its result is the same alone and in
+// any mix of requests.
+
+struct ServeRequest {
+    int grid = 0;
+    int kind = 0;  // 0: unit-slip columns, 1: forward solve
+    std::vector<int64_t> cols;
+    std::vector<double> fields;  // kind 1: slip fields on the face grid
+    int ncols = 0, pending = 0;
+    std::vector<double> result;  // 2nf x ncols, column-major
+    std::vector<int> iters;
+    int unconverged = 0;
+    bool done = false;
+};
+
+struct ServeGrid {
+    eqd_op* op = nullptr;
+    eqd_solver* s = nullptr;
+    int ntiles = 0, slots = 0;
+    std::vector<int64_t> slot_req, colid;
+    std::vector<int> slot_idx;
+    DevMem arena, mask_fresh, mask_unit, mask_retire;
+    std::deque<int64_t> queue;
+    std::vector<ColState> hcs;
+    int64_t busy = 0;
+    ~ServeGrid() {
+        if (s) eqd_solver_destroy(s);
+        if (op) eqd_operator_destroy(op);
+    }
+};
+
+struct eqd_server {
+    eqd_cg_opts o{};
+    size_t budget = 0, used = 0;
+    std::vector<std::unique_ptr<ServeGrid>> grids;
+    std::map<int64_t, ServeRequest> reqs;
+    std::deque<int64_t> completed;
+    int64_t next_id = 1;
+};
+
+namespace {
+
+void serve_check_grid(const eqd_server* sv, int grid) {
+    if (grid < 0 || grid >= (int)sv->grids.size()) throw Error(EQD_ERR_ARG, "unknown grid id");
+}
+
+// Retire finished columns of one grid, admit queued requests into free lanes, replay once.
+int64_t serve_grid_step(eqd_server* sv, ServeGrid& G) {
+    eqd_solver* s = G.s;
+    Workspace& w = *s->ws;
+    const eqd_op& op = *G.op;
+    const int n2 = 2 * op.nf;
+    std::vector<int> retire(G.slots, 0), fresh(G.slots, 0), unit(G.slots, 0);
+    bool any_retire = false, any_change = false;
+    if (G.busy > 0) {
+        HIPCHK(hipMemcpyAsync(G.hcs.data(), w.cs.p, G.slots * sizeof(ColState), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        for (int i = 0; i < G.slots; ++i)
+            if (G.slot_req[i] >= 0 && G.hcs[i].status != kActive) {
+                retire[i] = 1;
+                any_retire = true;
+            }
+    }
+    if (any_retire) {
+        HIPCHK(hipMemcpyAsync(G.mask_retire.p, retire.data(), G.slots * sizeof(int), hipMemcpyHostToDevice,
+                              s->stream));
+        with_k(64, [&](auto d) {
+            decltype(d)::fold(s, G.mask_retire.as<int>());
+            decltype(d)::extract(s, G.arena.as<double>(), n2, G.mask_retire.as<int>());
+        });
+        std::vector<int64_t> finished;
+        for (int i = 0; i < G.slots; ++i) {
+            if (!retire[i]) continue;
+            finished.push_back(G.slot_req[i]);
+            ServeRequest& r = sv->reqs.at(G.slot_req[i]);
+            const int j = G.slot_idx[i];
+            HIPCHK(hipMemcpyAsync(r.result.data() + (size_t)j * n2, G.arena.as<double>() + (size_t)i * n2, n2 * 8,
+                                  hipMemcpyDeviceToHost, s->stream));
+            const ColState& c = G.hcs[i];
+            r.iters[j] = c.iters;
+            s->st.solves += 1;
+            s->st.iterations += c.iters;
+            s->st.max_iterations = std::max<int64_t>(s->st.max_iterations, c.iters);
+            if (c.status != kSolved) {
+                r.unconverged += 1;
+                s->st.unconverged += 1;
+            }
+            G.slot_req[i] = -1;
+            G.colid[i] = -1;
+            G.busy -= 1;
+            fresh[i] = 1;  // re-initialised below: idle unless a new column takes the lane
+            any_change = true;
+        }
+        HIPCHK(hipStreamSynchronize(s->stream));  // the result copies have landed
+        for (int64_t id : finished) {
+            ServeRequest& r = sv->reqs.at(id);
+            if (--r.pending == 0) {
+                r.done = true;
+                sv->completed.push_back(id);
+            }
+        }
+    }
+    // admission, whole requests in arrival order
+    std::vector<int> freelist;
+    for (int i = 0; i < G.slots; ++i)
+        if (G.slot_req[i] < 0) freelist.push_back(i);
+    size_t fi = 0;
+    std::vector<std::pair<int, const double*>> fwd;
+    while (!G.queue.empty()) {
+        ServeRequest& r = sv->reqs.at(G.queue.front());
+        if ((size_t)r.ncols > freelist.size() - fi) break;
+        const int64_t id = G.queue.front();
+        G.queue.pop_front();
+        for (int j = 0; j < r.ncols; ++j) {
+            const int i = freelist[fi++];
+            G.slot_req[i] = id;
+            G.slot_idx[i] = j;
+            fresh[i] = 1;
+            if (r.kind == 0) {
+                G.colid[i] = r.cols[j];
+                unit[i] = 1;
+            } else {
+                G.colid[i] = (int64_t)1 << 40;  // not a unit column: fields come from the request
+                fwd.emplace_back(i, r.fields.data());
+            }
+            G.busy += 1;
+            any_change = true;
+        }
+    }
+    if (any_change) {
+        HIPCHK(hipMemcpyAsync(w.colid.p, G.colid.data(), G.slots * 8, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(G.mask_fresh.p, fresh.data(), G.slots * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(G.mask_unit.p, unit.data(), G.slots * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        const size_t fsz = 2 * (size_t)op.g.n23 * op.g.n23;
+        for (auto& f : fwd)
+            HIPCHK(hipMemcpyAsync(w.fields.as<double>() + (size_t)f.first * fsz, f.second, fsz * 8,
+                                  hipMemcpyHostToDevice, s->stream));
+        hipLaunchKernelGGL(k_unit_fields, dim3(G.slots), dim3(256), 0, s->stream, op.g.n23, op.i2lo, op.n2f, op.nf,
+                           G.slots, w.colid.as<int64_t>(), w.fields.as<double>(), G.mask_unit.as<int>());
+        HIPCHK(hipGetLastError());
+        with_k(64, [&](auto d) {
+            decltype(d)::rhs(s, G.mask_fresh.as<int>());
+            decltype(d)::init(s, G.mask_fresh.as<int>());
+        });
+        // host copies of request fields must outlive the async copies above
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    if (G.busy > 0) with_k(64, [&](auto d) { decltype(d)::replay(s); });
+    return G.busy;
 }
 
 }  // namespace
@@ -1000,7 +1197,8 @@ eqd_status eqd_build_rhs(eqd_op* op, const double* s2, const double* s3, double*
         DevMem df(f.size() * 8), db(n * 8), dx(n * 8), dr(n * 8), dp(n * 8);
         HIPCHK(hipMemcpy(df.p, f.data(), f.size() * 8, hipMemcpyHostToDevice));
         hipLaunchKernelGGL((k_rhs<double, 1>), dim3(kGroups, 1), dim3(kBlock), 0, 0, op->rv, op->NN, df.as<double>(), 1,
-                           db.as<double>(), dx.as<double>(), dr.as<double>(), dp.as<double>(), (float*)nullptr);
+                           db.as<double>(), dx.as<double>(), dr.as<double>(), dp.as<double>(), (float*)nullptr,
+                           (const int*)nullptr);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(b.data(), db.p, n * 8, hipMemcpyDeviceToHost));
         embed_full(*op, b.data(), rhs);
@@ -1097,6 +1295,175 @@ eqd_status eqd_fault_stiffness(eqd_solver* s, int mode, const int* cols, int nco
         DevMem dK((size_t)std::max<int64_t>(nout, 1) * ldK * 8);
         stiffness(s, mode, cols, ncols, dK.as<double>(), ldK);
         if (nout > 0) HIPCHK(hipMemcpy(K, dK.p, (size_t)nout * ldK * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+eqd_status eqd_server_create(const eqd_cg_opts* opts, size_t hbm_budget_bytes, eqd_server** out) {
+    return guard([&] {
+        if (!out) throw Error(EQD_ERR_ARG, "null argument");
+        check_device();
+        auto sv = std::make_unique<eqd_server>();
+        if (opts) sv->o = *opts;
+        else eqd_cg_opts_default(&sv->o);
+        if (sv->o.engine != EQD_ENGINE_BATCHED) throw Error(EQD_ERR_ARG, "the serving loop runs the batched engine");
+        sv->o.tile_k = 64;
+        sv->o.timing = 0;
+        sv->budget = hbm_budget_bytes;
+        if (sv->budget == 0) {
+            size_t freeb = 0, totb = 0;
+            HIPCHK(hipMemGetInfo(&freeb, &totb));
+            sv->budget = freeb / 2;
+        }
+        *out = sv.release();
+    });
+}
+
+eqd_status eqd_server_destroy(eqd_server* sv) {
+    return guard([&] { delete sv; });
+}
+
+eqd_status eqd_server_add_grid(eqd_server* sv, const eqd_grid_desc* g, int max_columns, int* grid_id, int* slots) {
+    return guard([&] {
+        if (!sv || !g) throw Error(EQD_ERR_ARG, "null argument");
+        auto G = std::make_unique<ServeGrid>();
+        eqd_status rc = eqd_operator_create(g, &G->op);
+        if (rc != EQD_OK) throw Error(rc, g_err);
+        rc = eqd_solver_create(G->op, &sv->o, &G->s);
+        if (rc != EQD_OK) throw Error(rc, g_err);
+        const size_t per = bytes_per_column(*G->op, sv->o.precision) + 2 * (size_t)G->op->nf * 8;
+        const size_t room = sv->budget > sv->used ? sv->budget - sv->used : 0;
+        int tiles = (int)std::min<size_t>(room / (per * 64), 1 << 20);
+        if (max_columns > 0) tiles = std::min(tiles, (max_columns + 63) / 64);
+        if (tiles < 1) throw Error(EQD_ERR_NOMEM, "the HBM budget left does not hold one 64-column tile of this grid");
+        ensure_ws(G->s, 64, tiles, sv->o.precision);
+        G->ntiles = tiles;
+        G->slots = tiles * 64;
+        sv->used += (size_t)G->slots * per;
+        G->slot_req.assign(G->slots, -1);
+        G->slot_idx.assign(G->slots, 0);
+        G->colid.assign(G->slots, -1);
+        G->hcs.resize(G->slots);
+        G->arena.alloc((size_t)G->slots * 2 * G->op->nf * 8);
+        G->mask_fresh.alloc(G->slots * sizeof(int));
+        G->mask_unit.alloc(G->slots * sizeof(int));
+        G->mask_retire.alloc(G->slots * sizeof(int));
+        Workspace& w = *G->s->ws;
+        std::vector<int64_t> opos(G->slots);
+        for (int i = 0; i < G->slots; ++i) opos[i] = i;  // each lane extracts into its arena column
+        HIPCHK(hipMemcpy(w.outpos.p, opos.data(), G->slots * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(w.colid.p, G->colid.data(), G->slots * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(w.fields.p, 0, w.fields.bytes));
+        HIPCHK(hipMemset(w.b.p, 0, w.b.bytes));
+        with_k(64, [&](auto d) { decltype(d)::init(G->s); });  // every lane idle, every tile off
+        HIPCHK(hipStreamSynchronize(G->s->stream));
+        if (grid_id) *grid_id = (int)sv->grids.size();
+        if (slots) *slots = G->slots;
+        sv->grids.push_back(std::move(G));
+    });
+}
+
+eqd_status eqd_server_submit_columns(eqd_server* sv, int grid, const int* cols, int ncols, int64_t* req_id) {
+    return guard([&] {
+        if (!sv || !cols || ncols < 1) throw Error(EQD_ERR_ARG, "a request needs at least one column");
+        serve_check_grid(sv, grid);
+        ServeGrid& G = *sv->grids[grid];
+        if (ncols > G.slots) throw Error(EQD_ERR_ARG, "request larger than the grid's lane pool");
+        const int n2 = 2 * G.op->nf;
+        ServeRequest r;
+        r.grid = grid;
+        r.kind = 0;
+        r.ncols = r.pending = ncols;
+        for (int j = 0; j < ncols; ++j) {
+            if (cols[j] < 0 || cols[j] >= n2) throw Error(EQD_ERR_ARG, "column index out of range");
+            r.cols.push_back(cols[j]);
+        }
+        r.result.assign((size_t)n2 * ncols, 0.0);
+        r.iters.assign(ncols, 0);
+        const int64_t id = sv->next_id++;
+        sv->reqs.emplace(id, std::move(r));
+        G.queue.push_back(id);
+        if (req_id) *req_id = id;
+    });
+}
+
+eqd_status eqd_server_submit_forward(eqd_server* sv, int grid, const double* s2, const double* s3, int64_t* req_id) {
+    return guard([&] {
+        if (!sv || !s2 || !s3) throw Error(EQD_ERR_ARG, "null argument");
+        serve_check_grid(sv, grid);
+        ServeGrid& G = *sv->grids[grid];
+        const int n2 = 2 * G.op->nf;
+        ServeRequest r;
+        r.grid = grid;
+        r.kind = 1;
+        r.ncols = r.pending = 1;
+        r.fields.assign(2 * (size_t)G.op->g.n23 * G.op->g.n23, 0.0);
+        slip_fields(*G.op, s2, s3, r.fields.data());
+        r.result.assign(n2, 0.0);
+        r.iters.assign(1, 0);
+        const int64_t id = sv->next_id++;
+        sv->reqs.emplace(id, std::move(r));
+        G.queue.push_back(id);
+        if (req_id) *req_id = id;
+    });
+}
+
+eqd_status eqd_server_step(eqd_server* sv, int64_t* lanes_busy, int64_t* queued) {
+    return guard([&] {
+        if (!sv) throw Error(EQD_ERR_ARG, "null server");
+        int64_t busy = 0, q = 0;
+        for (auto& G : sv->grids) {
+            busy += serve_grid_step(sv, *G);
+            q += (int64_t)G->queue.size();
+        }
+        if (lanes_busy) *lanes_busy = busy;
+        if (queued) *queued = q;
+    });
+}
+
+eqd_status eqd_server_completed(eqd_server* sv, int64_t* ids, int max, int* n) {
+    return guard([&] {
+        if (!sv || !n) throw Error(EQD_ERR_ARG, "null argument");
+        int k = 0;
+        while (k < max && !sv->completed.empty()) {
+            if (ids) ids[k] = sv->completed.front();
+            sv->completed.pop_front();
+            ++k;
+        }
+        *n = k;
+    });
+}
+
+eqd_status eqd_server_result(eqd_server* sv, int64_t id, double* out, int64_t ld, int* iters, int* unconverged) {
+    return guard([&] {
+        if (!sv) throw Error(EQD_ERR_ARG, "null server");
+        auto it = sv->reqs.find(id);
+        if (it == sv->reqs.end()) throw Error(EQD_ERR_ARG, "unknown request id");
+        ServeRequest& r = it->second;
+        if (!r.done) throw Error(EQD_ERR_ARG, "request not completed");
+        const int n2 = 2 * sv->grids[r.grid]->op->nf;
+        if (out) {
+            if (ld < n2) throw Error(EQD_ERR_ARG, "ld must be at least 2nf");
+            for (int j = 0; j < r.ncols; ++j)
+                std::memcpy(out + (size_t)j * ld, r.result.data() + (size_t)j * n2, n2 * 8);
+        }
+        if (iters) std::copy(r.iters.begin(), r.iters.end(), iters);
+        if (unconverged) *unconverged = r.unconverged;
+        sv->reqs.erase(it);
+    });
+}
+
+eqd_status eqd_server_grid_info(const eqd_server* sv, int grid, eqd_op_info* info, int* slots, double* hbm_bytes) {
+    return guard([&] {
+        if (!sv) throw Error(EQD_ERR_ARG, "null server");
+        serve_check_grid(sv, grid);
+        const ServeGrid& G = *sv->grids[grid];
+        if (info) {
+            const eqd_status rc = eqd_operator_info(G.op, info);
+            if (rc != EQD_OK) throw Error(rc, g_err);
+        }
+        if (slots) *slots = G.slots;
+        if (hbm_bytes)
+            *hbm_bytes = (double)G.slots * (double)(bytes_per_column(*G.op, sv->o.precision) + 2 * (size_t)G.op->nf * 8);
     });
 }
 

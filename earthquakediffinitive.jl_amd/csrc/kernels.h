@@ -449,12 +449,16 @@ __global__ __launch_bounds__(kBlock) void k_pupdate64(int NN, T* __restrict__ x,
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_xaccum(int NN, double* __restrict__ x, float* __restrict__ xc,
                                                    const ColState* __restrict__ cs, const int* __restrict__ tile_on,
-                                                   int all) {
+                                                   int all, const int* __restrict__ mask) {
     constexpr int NPB = kBlock / K;
     int tile, g;
     tile_group(tile, g);
     const int lane = threadIdx.x % K, slot = threadIdx.x / K;
-    if (!all && (!tile_on[tile] || !cs[(size_t)tile * K + lane].rel_flag)) return;
+    if (mask) {  // exactly the masked columns (lanes retired by the serving loop)
+        if (!mask[(size_t)tile * K + lane]) return;
+    } else if (!all && (!tile_on[tile] || !cs[(size_t)tile * K + lane].rel_flag)) {
+        return;
+    }
     const size_t off = (size_t)tile * NN * 3 * K;
     for (int base = g * NPB; base < NN; base += kGroups * NPB) {
         const int node = base + slot;
@@ -545,9 +549,10 @@ __global__ void k_finalize(int mode, int mixed, int rel_interval, ColState* __re
 // Unit-slip fields on the face grid (n23 x n23 per column, two directions):
 // column id < nf -> s2 = e_col, else s3 = e_(col - nf)   (src/FaultResponse.jl:249-253).
 __global__ void k_unit_fields(int n23, int i2lo, int n2f, int nf, int ncols_slots, const int64_t* __restrict__ colid,
-                              double* __restrict__ fields) {
+                              double* __restrict__ fields, const int* __restrict__ mask) {
     const int slot = blockIdx.x;
     if (slot >= ncols_slots) return;
+    if (mask && !mask[slot]) return;
     double* f = fields + (size_t)slot * 2 * n23 * n23;
     for (int i = threadIdx.x; i < 2 * n23 * n23; i += blockDim.x) f[i] = 0.0;
     __syncthreads();
@@ -573,12 +578,13 @@ __device__ __forceinline__ double tab_at(const Tab<double>& t, int row, int col)
 template <typename T, int K>
 __global__ __launch_bounds__(kBlock) void k_rhs(RhsView rv, int NN, const double* __restrict__ fields, int nslots,
                                                 double* __restrict__ b, double* __restrict__ x, T* __restrict__ r,
-                                                T* __restrict__ p, float* __restrict__ xc) {
+                                                T* __restrict__ p, float* __restrict__ xc, const int* __restrict__ mask) {
     constexpr int NPB = kBlock / K;
     int tile, g;
     tile_group(tile, g);
     const int lane = threadIdx.x % K, slot = threadIdx.x / K;
     const int col = tile * K + lane;
+    if (mask && !mask[col]) return;  // only the masked (newly admitted) columns
     const int n23 = rv.n23;
     const double* s2 = fields + (size_t)col * 2 * n23 * n23;
     const double* s3 = s2 + (size_t)n23 * n23;
@@ -621,25 +627,29 @@ __global__ __launch_bounds__(kBlock) void k_rhs(RhsView rv, int NN, const double
 template <int K>
 __global__ void k_init_state(ColState* __restrict__ cs, const double* __restrict__ partial,
                              const int64_t* __restrict__ colid, double rtol, double atol, int64_t itmax,
-                             int* __restrict__ tile_on) {
+                             int* __restrict__ tile_on, const int* __restrict__ mask) {
     const int tile = blockIdx.x, lane = threadIdx.x;
     ColState& st = cs[(size_t)tile * K + lane];
-    const double s = sum_partials<K>(partial, tile, lane);
-    st.col = colid[(size_t)tile * K + lane];
-    st.rr = s;
-    st.pnorm2 = s;
-    st.bnorm = sqrt(s);
-    st.eps = atol + rtol * st.bnorm;
-    st.alpha = st.beta = 0.0;
-    st.iters = 0;
-    st.since_rel = 0;
-    st.rel_flag = 0;
-    st.xpend = 0;
-    st.pad_ = 0;
-    st.itmax = itmax;
-    if (st.col < 0) st.status = kIdle;
-    else if (s == 0.0 || st.bnorm <= st.eps) st.status = kSolved;  // x = 0 is the answer
-    else st.status = kActive;
+    // Columns outside the mask (already in flight) keep their state. No early return: the
+    // tile's activity below is one reduction over every lane of the block.
+    if (!mask || mask[(size_t)tile * K + lane]) {
+        const double s = sum_partials<K>(partial, tile, lane);
+        st.col = colid[(size_t)tile * K + lane];
+        st.rr = s;
+        st.pnorm2 = s;
+        st.bnorm = sqrt(s);
+        st.eps = atol + rtol * st.bnorm;
+        st.alpha = st.beta = 0.0;
+        st.iters = 0;
+        st.since_rel = 0;
+        st.rel_flag = 0;
+        st.xpend = 0;
+        st.pad_ = 0;
+        st.itmax = itmax;
+        if (st.col < 0) st.status = kIdle;
+        else if (s == 0.0 || st.bnorm <= st.eps) st.status = kSolved;  // x = 0 is the answer
+        else st.status = kActive;
+    }
     const int on = __syncthreads_or(st.status == kActive);
     if (lane == 0) tile_on[tile] = on;
 }
@@ -672,11 +682,12 @@ __global__ __launch_bounds__(kBlock) void k_selfdot(int NN, int NX, int mface, c
 template <int K>
 __global__ void k_extract(ExtractView ev, int NN, const double* __restrict__ u, const double* __restrict__ fields,
                           const int64_t* __restrict__ colid, double* __restrict__ Kout, int64_t ldK,
-                          int64_t col_offset) {
+                          int64_t col_offset, const int* __restrict__ mask) {
     const int tile = blockIdx.y;
     const int lane = threadIdx.x % K;
     const int k = blockIdx.x * (blockDim.x / K) + threadIdx.x / K;  // Omega_f node
     if (k >= ev.nf) return;
+    if (mask && !mask[(size_t)tile * K + lane]) return;
     const int64_t col = colid[(size_t)tile * K + lane];
     if (col < 0) return;
     const int n23 = ev.n23, NX = ev.NX, NY = ev.NY;

@@ -31,6 +31,9 @@ EXPORTED = [
     "eqd_bp8_destroy", "eqd_bp8_sizes", "eqd_bp8_stiffness", "eqd_bp8_operators", "eqd_bp8_initial_state",
     "eqd_bp8_evaluate", "eqd_bp8_rhs", "eqd_bp8_stress_report", "eqd_bp8_reset_cache", "eqd_bp8_integrate",
     "eqd_bp8_saved_device", "eqd_bp8_saved_vmax",
+    "eqd_server_create", "eqd_server_destroy", "eqd_server_add_grid", "eqd_server_submit_columns",
+    "eqd_server_submit_forward", "eqd_server_step", "eqd_server_completed", "eqd_server_result",
+    "eqd_server_grid_info",
 ]
 
 INJECTION_GAUSSIAN, INJECTION_PEACEMAN = 0, 1
@@ -131,6 +134,16 @@ def lib():
                                         vp, vp, C.POINTER(BP8Stats)]
         L.eqd_bp8_saved_device.argtypes = [vp, C.POINTER(vp), i64p, i64p]
         L.eqd_bp8_saved_vmax.argtypes = [vp, dp, C.c_int64]
+        L.eqd_server_create.argtypes = [C.POINTER(CgOpts), C.c_size_t, C.POINTER(vp)]
+        L.eqd_server_destroy.argtypes = [vp]
+        L.eqd_server_add_grid.argtypes = [vp, C.POINTER(GridDesc), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.eqd_server_submit_columns.argtypes = [vp, C.c_int, vp, C.c_int, i64p]
+        L.eqd_server_submit_forward.argtypes = [vp, C.c_int, dp, dp, i64p]
+        L.eqd_server_step.argtypes = [vp, i64p, i64p]
+        L.eqd_server_completed.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
+        L.eqd_server_result.argtypes = [vp, C.c_int64, vp, C.c_int64, vp, C.POINTER(C.c_int)]
+        L.eqd_server_grid_info.argtypes = [vp, C.c_int, C.POINTER(OpInfo), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_double)]
         L.eqd_version.restype = C.c_char_p
         L.eqd_last_error.argtypes = [C.c_char_p, C.c_size_t]
         L.eqd_last_error.restype = C.c_size_t

@@ -225,6 +225,38 @@ eqd_status eqd_bp8_saved_device(const eqd_bp8* m, const double** us_dev, int64_t
  * in save order as write_outputs does (src/BP8.jl:628). */
 eqd_status eqd_bp8_saved_vmax(eqd_bp8* m, double* vmax, int64_t n);
 
+/* ------------------------------------------------------------------------------
+ * Serving loop (no counterpart in the reference, whose K build is a batch job): stiffness
+ * column and forward-solve requests on several grids, admitted into resident 64-column
+ * tile pools as lanes free up (continuous batching), each column's result independent of
+ * the mix it ran in.
+ * ------------------------------------------------------------------------------ */
+typedef struct eqd_server eqd_server;
+
+/* opts: CG options of every grid (batched engine; tile_k is forced to 64). hbm_budget_bytes
+ * bounds the tile pools of all grids together (0 = half of the free device memory). */
+eqd_status eqd_server_create(const eqd_cg_opts* opts, size_t hbm_budget_bytes, eqd_server** out);
+eqd_status eqd_server_destroy(eqd_server* sv);
+/* Makes a grid resident: operator tables, a pool of 64-column tiles (as many as the
+ * remaining budget holds, at most ceil(max_columns / 64) when max_columns > 0). */
+eqd_status eqd_server_add_grid(eqd_server* sv, const eqd_grid_desc* g, int max_columns, int* grid_id, int* slots);
+/* Queues a request for the unit-slip columns cols[0..ncols) (0-based, < 2nf) of K, i.e.
+ * fault_stiffness restricted to those columns (src/FaultResponse.jl:229-287). */
+eqd_status eqd_server_submit_columns(eqd_server* sv, int grid, const int* cols, int ncols, int64_t* req_id);
+/* Queues a forward solve: shear_traction(fe, s2, s3) (src/FaultResponse.jl:194-211). */
+eqd_status eqd_server_submit_forward(eqd_server* sv, int grid, const double* s2, const double* s3, int64_t* req_id);
+/* One step: retire finished columns, admit queued requests into free lanes, run one graph
+ * replay of CG iterations on every grid with work. Returns the lanes in flight and the
+ * requests still queued. */
+eqd_status eqd_server_step(eqd_server* sv, int64_t* lanes_busy, int64_t* queued);
+/* Pops up to max completed request ids. */
+eqd_status eqd_server_completed(eqd_server* sv, int64_t* ids, int max, int* n);
+/* Result of a completed request (then forgotten): 2nf x ncols column-major [dtau2; dtau3]
+ * (ld >= 2nf), CG iterations per column, count of columns that did not converge. */
+eqd_status eqd_server_result(eqd_server* sv, int64_t id, double* out, int64_t ld, int* iters, int* unconverged);
+/* Operator info, lane count and device bytes of a grid's pool. */
+eqd_status eqd_server_grid_info(const eqd_server* sv, int grid, eqd_op_info* info, int* slots, double* hbm_bytes);
+
 /* Version and last error. */
 const char* eqd_version(void);
 size_t eqd_last_error(char* buf, size_t len);
