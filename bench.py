@@ -11,7 +11,8 @@ against the batched FP64 K and against the CPU oracle on sampled columns, the
 stencil kernel's average launch duration (HIP events on the solver's stream) for the
 roofline, the CPU oracle's threaded build on a column sample, and the BP8 time loop
 (30 days of BP8-QD-GS at dz = 50 m with a device-built Toeplitz K) beside the CPU oracle's
-integration of the same K.
+integration of the same K, and the serving loop under a seeded Poisson request stream
+(p50/p99 latency, columns/s).
 """
 import argparse
 import ctypes
@@ -126,6 +127,84 @@ def bench_bp8(args):
     return res
 
 
+def bench_serving(args):
+    """Serving loop under a seeded Poisson stream (mean args.serve_rate req/s for
+    args.serve_seconds s) over three grids with l_f = L_fault: (dz, L_fault, L_normal) =
+    (100, 800, 800) -- the legal form of (100, 800, 400), whose n1 = 5 < 9 -- (50, 800, 400) and
+    (25, 1600, 800), weights 0.5 / 0.35 / 0.15. A request is, with equal probability, k ~ U{1..64}
+    unit-slip columns at random nodes and directions, or a forward solve with a Gaussian slip bump
+    (amplitude 1e-3 m, width U[100, 400] m, centre uniform on Omega_f, random direction)."""
+    from eqd.serve import Server
+    mu, nu = 32.04e9, 0.25
+    lam = 2 * mu * nu / (1 - 2 * nu)
+    specs = [(100.0, 800.0, 800.0), (50.0, 800.0, 400.0), (25.0, 1600.0, 800.0)]
+    weights = np.array([0.5, 0.35, 0.15])
+    sv = Server(precision="mixed")
+    gids, axes = [], []
+    t0 = time.perf_counter()
+    for dz, Lf, Ln in specs:
+        gids.append(sv.add_grid(eqd.bp8_grid_pair(dz, Lf, Ln), lam, mu, l_f=Lf, max_columns=args.serve_lanes))
+        n2f = sv.grids[-1]["n2f"]
+        axes.append(np.linspace(-Lf, Lf, n2f))
+    t_setup = time.perf_counter() - t0
+    rng = np.random.default_rng(20261203)
+    n_req = rng.poisson(args.serve_rate * args.serve_seconds)
+    arrivals = np.sort(rng.uniform(0.0, args.serve_seconds, n_req))
+    reqs = []
+    for a in arrivals:
+        gi = int(rng.choice(3, p=weights))
+        nf = sv.grids[gi]["nf"]
+        if rng.random() < 0.5:
+            k = int(rng.integers(1, 65))
+            reqs.append((a, gi, "columns", rng.choice(2 * nf, size=k, replace=False), None))
+        else:
+            x = axes[gi]
+            X, Y = np.meshgrid(x, x, indexing="ij")
+            w = rng.uniform(100.0, 400.0)
+            c2, c3 = rng.uniform(x[0], x[-1], 2)
+            bump = (1e-3 * np.exp(-((X - c2) ** 2 + (Y - c3) ** 2) / (2 * w * w))).ravel(order="F")
+            zero = np.zeros(nf)
+            reqs.append((a, gi, "forward", None, (bump, zero) if rng.random() < 0.5 else (zero, bump)))
+    info = {}
+    pending = {}
+    lat = [[] for _ in specs]
+    cols_done = 0
+    i = 0
+    start = time.perf_counter()
+    while i < len(reqs) or pending:
+        now = time.perf_counter() - start
+        while i < len(reqs) and reqs[i][0] <= now:
+            a, gi, kind, cols, slip = reqs[i]
+            rid = sv.submit_columns(gids[gi], cols) if kind == "columns" else sv.submit_forward(gids[gi], *slip)
+            pending[rid] = (a, gi, 1 if cols is None else len(cols))
+            i += 1
+        busy, queued = sv.step()
+        for rid in sv.completed():
+            a, gi, nc = pending.pop(rid)
+            sv.result(rid, gids[gi], nc)
+            lat[gi].append(time.perf_counter() - start - a)
+            cols_done += nc
+        if busy == 0 and queued == 0 and i < len(reqs):
+            time.sleep(max(0.0, min(reqs[i][0] - (time.perf_counter() - start), 0.002)))
+        if time.perf_counter() - start > args.serve_seconds * 20 + 120:
+            raise RuntimeError("serving run did not drain")
+    wall = time.perf_counter() - start
+    allat = np.concatenate([np.array(l) for l in lat if l]) * 1e3
+    info.update(offered_req_per_s=args.serve_rate, duration_s=args.serve_seconds, requests=len(reqs),
+                wall_s=round(wall, 2), columns=cols_done, columns_per_s=round(cols_done / wall, 1),
+                p50_ms=round(float(np.percentile(allat, 50)), 1), p99_ms=round(float(np.percentile(allat, 99)), 1),
+                mean_ms=round(float(allat.mean()), 1), setup_s=round(t_setup, 2),
+                hbm_pool_GB=round(sum(g["hbm_bytes"] for g in sv.grids) / 1e9, 2),
+                lanes=[g["slots"] for g in sv.grids])
+    info["by_grid"] = {f"dz{int(d)}_Lf{int(f)}_Ln{int(n)}": (
+        dict(requests=len(l), p50_ms=round(float(np.percentile(np.array(l) * 1e3, 50)), 1),
+             p99_ms=round(float(np.percentile(np.array(l) * 1e3, 99)), 1)) if l else dict(requests=0))
+        for (d, f, n), l in zip(specs, lat)}
+    print(f"bench: serving {len(reqs)} requests in {wall:.1f} s: p50 {info['p50_ms']} ms, p99 {info['p99_ms']} ms, "
+          f"{info['columns_per_s']} columns/s", file=sys.stderr, flush=True)
+    return info
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +222,11 @@ def main():
     ap.add_argument("--no_cpu", action="store_true")
     ap.add_argument("--no_extras", action="store_true", help="only the timed mixed-precision steps")
     ap.add_argument("--no_bp8", action="store_true", help="skip the BP8 time-loop measurement")
+    ap.add_argument("--no_serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve_seconds", type=float, default=10.0)
+    ap.add_argument("--serve_rate", type=float, default=10.0,
+                    help="mean request rate; 50 req/s over this grid mix exceeds one GPU (see DESIGN.md)")
+    ap.add_argument("--serve_lanes", type=int, default=512, help="lanes per grid in the serving pools")
     ap.add_argument("--copy_calib_mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (3 times) and exit: the "
                          "reference kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -268,6 +352,8 @@ def main():
     Kbuf.free()
     if not args.no_extras and not args.no_bp8:
         out["bp8_30day_gs"] = bench_bp8(args)
+    if not args.no_extras and not args.no_serve:
+        out["serving"] = bench_serving(args)
     print(json.dumps(out), flush=True)
 
 

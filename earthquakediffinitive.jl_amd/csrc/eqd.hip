@@ -855,7 +855,10 @@ struct ServeGrid {
     std::deque<int64_t> queue;
     std::vector<ColState> hcs;
     int64_t busy = 0;
+    hipEvent_t ev = nullptr;  // recorded after each replay: the grid steps again once it has passed
+    bool inflight = false;
     ~ServeGrid() {
+        if (ev) (void)hipEventDestroy(ev);
         if (s) eqd_solver_destroy(s);
         if (op) eqd_operator_destroy(op);
     }
@@ -877,7 +880,15 @@ void serve_check_grid(const eqd_server* sv, int grid) {
 }
 
 // Retire finished columns of one grid, admit queued requests into free lanes, replay once.
+// A grid whose previous replay is still running is left alone (grids run on their own
+// streams, so a large grid does not hold back the steps of small ones).
 int64_t serve_grid_step(eqd_server* sv, ServeGrid& G) {
+    if (G.inflight) {
+        const hipError_t q = hipEventQuery(G.ev);
+        if (q == hipErrorNotReady) return G.busy;
+        HIPCHK(q);
+        G.inflight = false;
+    }
     eqd_solver* s = G.s;
     Workspace& w = *s->ws;
     const eqd_op& op = *G.op;
@@ -977,7 +988,11 @@ int64_t serve_grid_step(eqd_server* sv, ServeGrid& G) {
         // host copies of request fields must outlive the async copies above
         HIPCHK(hipStreamSynchronize(s->stream));
     }
-    if (G.busy > 0) with_k(64, [&](auto d) { decltype(d)::replay(s); });
+    if (G.busy > 0) {
+        with_k(64, [&](auto d) { decltype(d)::replay(s); });
+        HIPCHK(hipEventRecord(G.ev, s->stream));
+        G.inflight = true;
+    }
     return G.busy;
 }
 
@@ -1347,6 +1362,7 @@ eqd_status eqd_server_add_grid(eqd_server* sv, const eqd_grid_desc* g, int max_c
         G->mask_fresh.alloc(G->slots * sizeof(int));
         G->mask_unit.alloc(G->slots * sizeof(int));
         G->mask_retire.alloc(G->slots * sizeof(int));
+        HIPCHK(hipEventCreateWithFlags(&G->ev, hipEventDisableTiming));
         Workspace& w = *G->s->ws;
         std::vector<int64_t> opos(G->slots);
         for (int i = 0; i < G->slots; ++i) opos[i] = i;  // each lane extracts into its arena column
