@@ -244,6 +244,8 @@ struct eqd_solver {
     double last_ms = 0;
     int64_t last_launches = 0, last_tile_iters = 0;
     std::vector<int> last_iters;  // per column of the last batch (slot order)
+    int tuned_k[2] = {0, 0};      // autotuned tile width per precision (0 = not yet tuned)
+    int last_k = 0;               // tile width of the last build
 };
 
 void eqd::host::solver_fault_dims(const eqd_solver* s, int& nf, int& n2f, int& n3f) {
@@ -453,6 +455,24 @@ struct Engine {
         return gi;
     }
 
+    // Device time (ms) of `iters` CG iterations on the current workspace (autotuning).
+    static double trial(eqd_solver* s, int iters) {
+        hipStream_t st = s->stream;
+        for (int i = 0; i < 2; ++i) iteration(s, st);  // warm-up
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, st));
+        for (int i = 0; i < iters; ++i) iteration(s, st);
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return ms;
+    }
+
     // One replay of the iteration graph (the serving loop's step).
     static void replay(eqd_solver* s) {
         ensure_graph(s);
@@ -629,6 +649,9 @@ struct Dispatch {
         if (s->ws->prec == EQD_PREC_MIXED) Engine<float, K>::replay(s);
         else Engine<double, K>::replay(s);
     }
+    static double trial(eqd_solver* s, int iters) {
+        return s->ws->prec == EQD_PREC_MIXED ? Engine<float, K>::trial(s, iters) : Engine<double, K>::trial(s, iters);
+    }
     static void extract(eqd_solver* s, double* Kdev, int64_t ldK, const int* mask = nullptr) {
         Engine<double, K>::extract(s, Kdev, ldK, mask);
     }
@@ -646,11 +669,10 @@ void with_k(int K, F&& f) {
     }
 }
 
-int effective_k(const eqd_solver* s, int ncols) {
+// Tile width of a build: 1 for the plain engine, the requested width, or 0 (autotune).
+int effective_k(const eqd_solver* s, int) {
     if (s->o.engine == EQD_ENGINE_PLAIN) return 1;
-    int K = s->o.tile_k;
-    if (K == 0) K = ncols >= 64 ? 64 : (ncols >= 16 ? 16 : 1);
-    return K;
+    return s->o.tile_k;
 }
 
 // Solves a batch of columns whose slip fields are produced by `fill` into
@@ -659,7 +681,6 @@ template <typename Fill>
 void run_columns(eqd_solver* s, const std::vector<int64_t>& colid, const std::vector<int64_t>& outpos, Fill&& fill,
                  double* Kdev, int64_t ldK) {
     const int n = (int)colid.size();
-    const int K = effective_k(s, n);
     const int prec = s->o.engine == EQD_ENGINE_PLAIN ? EQD_PREC_FP64 : s->o.precision;
     int per = s->o.max_cols_in_flight > 0 ? s->o.max_cols_in_flight : n;
     {  // bound by device memory
@@ -669,6 +690,40 @@ void run_columns(eqd_solver* s, const std::vector<int64_t>& colid, const std::ve
         per = (int)std::max<size_t>(1, std::min<size_t>((size_t)per, cap));
     }
     if (s->o.engine == EQD_ENGINE_PLAIN) per = 1;
+    int K = effective_k(s, n);
+    if (K == 0) {  // autotune on this batch: 16 CG iterations of its first columns per candidate width
+        int& tuned = s->tuned_k[prec == EQD_PREC_MIXED ? 1 : 0];
+        if (!tuned) {
+            const int nb = std::min({n, per, 40 * 64});
+            double best = 0.0;
+            for (int k : {8, 16, 32, 64}) {
+                const int nt = (nb + k - 1) / k;
+                ensure_ws(s, k, nt, prec);
+                Workspace& w = *s->ws;
+                const int sl = nt * k;
+                std::vector<int64_t> cid(sl, -1), opos(sl, -1);
+                for (int i = 0; i < sl && i < nb; ++i) {
+                    cid[i] = colid[i];
+                    opos[i] = outpos[i];
+                }
+                HIPCHK(hipMemcpyAsync(w.colid.p, cid.data(), sl * 8, hipMemcpyHostToDevice, s->stream));
+                HIPCHK(hipMemcpyAsync(w.outpos.p, opos.data(), sl * 8, hipMemcpyHostToDevice, s->stream));
+                fill(w, 0, sl);
+                double ms = 0.0;
+                with_k(k, [&](auto d) {
+                    decltype(d)::rhs(s);
+                    decltype(d)::init(s);
+                    ms = decltype(d)::trial(s, 16);
+                });
+                if (!tuned || ms < best) {
+                    best = ms;
+                    tuned = k;
+                }
+            }
+        }
+        K = tuned;
+    }
+    s->last_k = K;
     // tile_group balances any tile count over the 8 XCDs, so no padding tiles
     const int ntiles = (std::min(per, n) + K - 1) / K;
     ensure_ws(s, K, ntiles, prec);
@@ -1179,6 +1234,7 @@ eqd_status eqd_solver_report(const eqd_solver* s, eqd_report* r) {
         *r = s->st;
         r->mean_iterations = s->st.solves ? (double)s->st.iterations / (double)s->st.solves : 0.0;
         r->precond = s->o.precond;
+        r->tile_k = s->last_k;
     });
 }
 

@@ -156,7 +156,8 @@ def solver_report(s: CGSolver) -> dict:
     r = _lib.Report()
     check(lib().eqd_solver_report(s.handle, C.byref(r)))
     return dict(kind="cg", precond=s.precond, solves=r.solves, iterations=r.iterations,
-                mean_iterations=r.mean_iterations, max_iterations=r.max_iterations, unconverged=r.unconverged)
+                mean_iterations=r.mean_iterations, max_iterations=r.max_iterations, unconverged=r.unconverged,
+                tile_k=r.tile_k)
 
 
 class FaultElasticity:

@@ -70,7 +70,8 @@ class CgOpts(C.Structure):
 
 class Report(C.Structure):
     _fields_ = [("solves", C.c_int64), ("iterations", C.c_int64), ("max_iterations", C.c_int64),
-                ("unconverged", C.c_int64), ("mean_iterations", C.c_double), ("precond", C.c_int)]
+                ("unconverged", C.c_int64), ("mean_iterations", C.c_double), ("precond", C.c_int),
+                ("tile_k", C.c_int)]
 
 
 class EqdError(RuntimeError):

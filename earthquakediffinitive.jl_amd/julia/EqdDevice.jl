@@ -68,8 +68,9 @@ mutable struct Report
     unconverged::Int64
     mean_iterations::Cdouble
     precond::Cint
+    tile_k::Cint
 end
-Report() = Report(0, 0, 0, 0, 0.0, 0)
+Report() = Report(0, 0, 0, 0, 0.0, 0, 0)
 
 function check(rc::Cint)
     rc == EQD_OK && return nothing
@@ -157,7 +158,7 @@ function solver_report(s::DeviceCGSolver)
     check(ccall((:eqd_solver_report, libeqd), Cint, (Ptr{Cvoid}, Ref{Report}), s.handle, r))
     return (; solves=r.solves, iterations=r.iterations, mean_iterations=r.mean_iterations,
               max_iterations=r.max_iterations, unconverged=r.unconverged,
-              precond=r.precond == 0 ? :none : :jacobi)
+              precond=r.precond == 0 ? :none : :jacobi, tile_k=r.tile_k)
 end
 
 function shear_traction(fe::DeviceFaultElasticity, s2, s3)

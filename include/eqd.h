@@ -67,7 +67,8 @@ typedef struct {
     int precond;              /* EQD_PRECOND_NONE only; JACOBI is rejected (measured 0.92x in the reference) */
     int precision;            /* EQD_PREC_FP64 or EQD_PREC_MIXED (FP32 CG inside FP64 reliable updates) */
     int engine;               /* EQD_ENGINE_BATCHED (device-side alpha/beta) or _PLAIN (host alpha/beta, one column) */
-    int tile_k;               /* columns interleaved per tile: 1, 8, 16, 32 or 64 (0 = autotuned) */
+    int tile_k;               /* columns interleaved per tile: 1, 8, 16, 32 or 64; 0 = autotuned: the first
+                                 build times 16 CG iterations of its own first batch at each width */
     int max_cols_in_flight;   /* columns resident at once (0 = all, bounded by memory) */
     int reliable_interval;    /* mixed precision: iterations between FP64 residual refreshes (0 = 32) */
     int graph_iters;          /* iterations captured per HIP graph replay (0 = 8) */
@@ -78,6 +79,7 @@ typedef struct {
     int64_t solves, iterations, max_iterations, unconverged;
     double mean_iterations;
     int precond;
+    int tile_k;               /* tile width of the last build (the autotuned one when tile_k was 0) */
 } eqd_report;
 
 void eqd_cg_opts_default(eqd_cg_opts* o);

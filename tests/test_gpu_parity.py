@@ -178,3 +178,16 @@ def test_jacobi_rejected_unknown_precond_errors():
         pair("cube11", precond="jacobi")
     with pytest.raises(eqd.EqdError):
         pair("cube11", precond="ilu")
+
+
+def test_autotuned_tile_width():
+    """tile_k = 0: the first build times CG iterations at k in {8, 16, 32, 64} on its own batch,
+    keeps the fastest, and K is unchanged to the 1e-8 bar."""
+    fe, o = pair("ring", precision="mixed", tile_k=0)
+    K = eqd.fault_stiffness(fe)
+    k = eqd.solver_report(fe.rs)["tile_k"]
+    assert k in (8, 16, 32, 64)
+    assert rel(K, oracle_K("ring")) <= 1e-8
+    K2 = eqd.fault_stiffness(fe)  # tuned width is cached
+    assert eqd.solver_report(fe.rs)["tile_k"] == k
+    assert np.array_equal(K, K2)
