@@ -417,6 +417,311 @@ __global__ void k_reset_cache(Dev d) {
     if (i == 0) d.fail[0] = 0;
 }
 
+// ---------------------------------------------------------------------------------
+// Ensemble: M members (per-member Q0, a, sigma0) integrated in lockstep Tsit5 stages with
+// per-member step control. The stiffness product of a stage is one GEMM D = K S over all
+// members (S = the members' slip vectors), done on the FP64 matrix cores.
+
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+
+// D[i + ldD m] = sum_k A[i lda + k] B[k + ldB m] (A row-major, B and D column-major), FP64,
+// v_mfma_f64_16x16x4f64: lane l feeds A[l % 16][l / 16] and B[l / 16][l % 16] and holds
+// C[4 j + l / 16][l % 16] in element j. Block tile 64 x 64 (4 waves of 32 x 32), k chunks of
+// 16 staged in LDS.
+constexpr int kGemmBM = 64, kGemmBN = 64, kGemmBK = 16;
+__global__ __launch_bounds__(256) void k_dgemm_mfma(int nr, int nm, int nk, const double* __restrict__ A,
+                                                    int64_t lda, const double* __restrict__ B, int64_t ldB,
+                                                    double* __restrict__ D, int64_t ldD) {
+    __shared__ double As[kGemmBK][kGemmBM + 2];
+    __shared__ double Bs[kGemmBK][kGemmBN + 2];
+    const int i0 = blockIdx.x * kGemmBM, m0 = blockIdx.y * kGemmBN;
+    const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
+    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+    dvec4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < nk; k0 += kGemmBK) {
+#pragma unroll
+        for (int q = t; q < kGemmBM * kGemmBK; q += 256) {
+            const int r = q / kGemmBK, kk = q % kGemmBK;
+            const int gi = i0 + r, gk = k0 + kk;
+            As[kk][r] = (gi < nr && gk < nk) ? A[(size_t)gi * lda + gk] : 0.0;
+        }
+#pragma unroll
+        for (int q = t; q < kGemmBN * kGemmBK; q += 256) {
+            const int c = q / kGemmBK, kk = q % kGemmBK;
+            const int gm = m0 + c, gk = k0 + kk;
+            Bs[kk][c] = (gm < nm && gk < nk) ? B[(size_t)gk + (size_t)ldB * gm] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGemmBK; kk += 4) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const double av = As[kk + lane / 16][wr + a * 16 + lane % 16];
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const double bv = Bs[kk + lane / 16][wc + b * 16 + lane % 16];
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[a][b], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = i0 + wr + a * 16 + 4 * j + lane / 16, col = m0 + wc + b * 16 + lane % 16;
+                if (row < nr && col < nm) D[(size_t)row + (size_t)ldD * col] = acc[a][b][j];
+            }
+}
+
+struct EnsDev {
+    Consts c;              // shared constants (a, sigma0, src_rate, Q0 overridden per member)
+    const double* Lx;
+    const double* src;
+    const unsigned char* active;
+    const double* pa;      // per-member a
+    const double* psig;    // per-member sigma0
+    const double* pQ0;     // per-member Q0 (well injection rate)
+    const double* psrc;    // per-member Gaussian source amplitude (Q0 / L_fwid) / (beta phi)
+    int M, ldU;
+    double* Vprev;         // nf x M
+    double* Vm;            // nf x M (|V| of the last evaluation)
+    int* fail;             // M: Newton failure flags
+    double* k;             // 7 x ldU x M
+    double* uprev;         // ldU x M
+    double* unew;
+    double* partial;       // blocks x M
+    Ctl* ctl;              // M
+    double* saves;         // (M x nsave_max) x ldU
+    int64_t nsave_max;
+};
+
+// One rhs! per (node, member): thread i over nodes, member m = blockIdx.y. D holds K y
+// (rows 0 .. 2nf-1 of each member column, leading dimension ldD).
+__global__ __launch_bounds__(256) void k_ens_stage(EnsDev d, int stage, const double* __restrict__ y, int64_t ldY,
+                                                   const double* __restrict__ Dk, int64_t ldD,
+                                                   double* __restrict__ kout, double* __restrict__ ynext,
+                                                   const double* __restrict__ t_fixed) {
+    const Consts& c = d.c;
+    const int m = blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const Ctl& cm = d.ctl[m];
+    const bool run = stage == 0 || !cm.done;
+    double ssum = 0.0;
+    if (run && i < c.nf) {
+        const double dt = stage != 0 ? cm.dt : 0.0;
+        const double t = stage != 0 ? cm.t + kC[stage] * dt : t_fixed[m];
+        const size_t um = (size_t)m * d.ldU;
+        const double* ym = y + (size_t)m * ldY;
+        const double a = d.pa[m], sigma0 = d.psig[m];
+        const double dtau2 = Dk[(size_t)i + (size_t)ldD * m], dtau3 = Dk[(size_t)c.nf + i + (size_t)ldD * m];
+        const int n = c.n2f, ia = i % n, ib = i / n;
+        const double* P = ym + 3 * c.nf;
+        double ap = 0.0;
+        for (int q = 0; q < n; ++q) ap += d.Lx[ia * n + q] * P[q + n * ib] + d.Lx[ib * n + q] * P[ia + n * q];
+        ap *= c.alpha;
+        const double phi = ym[2 * c.nf + i], p = P[i];
+        double v2 = 0.0, v3 = 0.0, vm = 0.0;
+        const double T2 = c.tau0_2 + dtau2, T3 = c.tau0_3 + dtau3;
+        const size_t nm = (size_t)m * c.nf + i;
+        if (d.active[i]) {
+            const double sig = fmax(sigma0 - p, c.sigma_min);
+            const double T = sqrt(T2 * T2 + T3 * T3);
+            if (T != 0.0) {
+                Consts cc = c;
+                cc.a = a;
+                double V;
+                if (!slip_rate(T, exp(phi), sig, cc, d.Vprev[nm], V)) d.fail[m] = 1;
+                v2 = V * T2 / T;
+                v3 = V * T3 / T;
+            }
+            vm = sqrt(v2 * v2 + v3 * v3);
+            d.Vprev[nm] = fmax(vm, 1e-30);
+        }
+        d.Vm[nm] = vm;
+        double f[5];
+        f[0] = v2;
+        f[1] = v3;
+        f[2] = exp(-phi) - vm / c.D_RS;
+        f[3] = ap;
+        int ne = 4;
+        const int idx[5] = {i, c.nf + i, 2 * c.nf + i, 3 * c.nf + i, 4 * c.nf};
+        if (c.injection == 0) {
+            f[3] += (t < c.t_off ? d.psrc[m] : 0.0) * d.src[i];
+        } else if (i == c.well_cell) {
+            const double pw = ym[4 * c.nf];
+            const double transfer = c.WI * (pw - p);
+            f[3] += transfer / c.S_e;
+            f[4] = ((t < c.t_off ? d.pQ0[m] : 0.0) - transfer) / c.S_well;
+            ne = 5;
+        }
+        for (int q = 0; q < ne; ++q) {
+            const int e = idx[q];
+            const double fe = f[q];
+            kout[um + e] = fe;
+            if (stage >= 2 && stage <= 6) {
+                double s = 0.0;
+                for (int j = 0; j < stage - 1; ++j) s += kA[stage + 1][j] * d.k[(size_t)j * d.ldU * d.M + um + e];
+                s += kA[stage + 1][stage - 1] * fe;
+                ynext[um + e] = d.uprev[um + e] + dt * s;
+            } else if (stage == 7) {
+                double s = 0.0;
+                for (int j = 0; j < 6; ++j) s += kBt[j] * d.k[(size_t)j * d.ldU * d.M + um + e];
+                s += kBt[6] * fe;
+                const double sc = abstol_of(c, e) + fmax(fabs(d.uprev[um + e]), fabs(d.unew[um + e])) * cm.reltol;
+                const double r = dt * s / sc;
+                ssum += r * r;
+            }
+        }
+    }
+    if (stage == 7) {  // per-(block, member) partial of the squared scaled errors, fixed order
+        __shared__ double red[256];
+        red[threadIdx.x] = ssum;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) d.partial[(size_t)m * gridDim.x + blockIdx.x] = red[0];
+    }
+}
+
+// Per-member controller: one thread per member, the same logic as k_bp8_control.
+__global__ void k_ens_control(EnsDev d, int nblocks) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= d.M) return;
+    Ctl& s = d.ctl[m];
+    if (s.done) {
+        s.skip_commit = 1;
+        return;
+    }
+    double acc = 0.0;
+    for (int b = 0; b < nblocks; ++b) acc += d.partial[(size_t)m * nblocks + b];
+    const double beta1 = 7.0 / 50.0, beta2 = 2.0 / 25.0, qmin = 0.2, qmax = 10.0, gamma = 0.9;
+    const double EEst = sqrt(acc / d.c.nstate);
+    s.EEst = EEst;
+    s.nrhs += 6;
+    s.iters += 1;
+    s.skip_commit = 0;
+    s.end_save = 0;
+    s.save_count = 0;
+    double q;
+    if (EEst == 0.0) {
+        q = 1.0 / qmax;
+    } else {
+        s.q11 = pow(EEst, beta1);
+        q = s.q11 / pow(s.qold, beta2);
+        q = fmax(1.0 / qmax, fmin(1.0 / qmin, q / gamma));
+    }
+    s.t_step = s.t;
+    s.dt_step = s.dt;
+    if (d.fail[m]) {
+        s.done = 1;
+        s.retcode = 2;
+        s.accepted = 0;
+        s.skip_commit = 1;
+        return;
+    }
+    if (EEst <= 1.0) {
+        const double tnew = s.clipped ? s.tstop : s.t + s.dt;
+        s.t_new = tnew;
+        s.accepted = 1;
+        s.naccept += 1;
+        long long cnt = 0;
+        while (s.nsaved + cnt < s.nsave_max && s.t0 + (double)(s.nsaved + cnt) * s.saveat <= tnew + 1e-9 * s.saveat)
+            ++cnt;
+        s.save_first = (int)s.nsaved;
+        s.save_count = (int)cnt;
+        s.nsaved += cnt;
+        s.t = tnew;
+        s.qold = fmax(EEst, 1e-4);
+        s.dt = s.dt / q;
+        if (s.t >= s.tf) {
+            const double last = s.t0 + (double)(s.nsaved - 1) * s.saveat;
+            if (s.nsaved < s.nsave_max && fabs(last - s.tf) > 1e-9 * s.saveat) {
+                s.end_save = 1;
+                s.nsaved += 1;
+            }
+        }
+    } else {
+        s.accepted = 0;
+        s.nreject += 1;
+        s.dt = s.dt / fmin(1.0 / qmin, s.q11 / gamma);
+    }
+    if (s.t >= s.tf) {
+        s.done = 1;
+        s.retcode = 0;
+        return;
+    }
+    if (s.iters >= s.maxiters) {
+        s.done = 1;
+        s.retcode = 1;
+        return;
+    }
+    const double tstop = (d.c.t_off > s.t && d.c.t_off < s.tf) ? d.c.t_off : s.tf;
+    s.tstop = tstop;
+    s.clipped = s.dt >= tstop - s.t;
+    if (s.clipped) s.dt = tstop - s.t;
+}
+
+// Per-(entry, member) commit: saves (interpolated or on-grid), FSAL, next y2.
+__global__ __launch_bounds__(256) void k_ens_commit(EnsDev d, double* __restrict__ y2, int init) {
+    const int m = blockIdx.y;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    const Ctl& s = d.ctl[m];
+    if (!init && s.skip_commit) return;
+    if (e >= d.c.nstate) return;
+    const size_t um = (size_t)m * d.ldU, stride = (size_t)d.ldU * d.M;
+    double* k1 = d.k + um;
+    double up = d.uprev[um + e];
+    if (!init && s.accepted) {
+        const double un = d.unew[um + e];
+        double* sv = d.saves + (size_t)m * d.nsave_max * d.ldU;
+        for (int q = 0; q < s.save_count; ++q) {
+            const long long id = s.save_first + q;
+            const double ts = s.t0 + (double)id * s.saveat;
+            double v;
+            if (fabs(ts - s.t_new) <= 1e-9 * s.saveat) {
+                v = un;
+            } else {
+                double b[7];
+                interp_weights((ts - s.t_step) / s.dt_step, b);
+                double acc = 0.0;
+                for (int j = 0; j < 7; ++j) acc += b[j] * d.k[(size_t)j * stride + um + e];
+                v = up + s.dt_step * acc;
+            }
+            sv[(size_t)id * d.ldU + e] = v;
+        }
+        if (s.end_save) sv[(size_t)(s.nsaved - 1) * d.ldU + e] = un;
+        up = un;
+        d.uprev[um + e] = un;
+        k1[e] = d.k[(size_t)6 * stride + um + e];
+    }
+    y2[um + e] = up + s.dt * kA21 * k1[e];
+}
+
+// max_i |V| per member of the last k_ens_stage evaluation (one block per member).
+__global__ void k_ens_vmax(const double* __restrict__ Vm, int nf, double* __restrict__ out, int64_t ldo) {
+    const int m = blockIdx.x;
+    __shared__ double red[256];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nf; i += 256) v = fmax(v, Vm[(size_t)m * nf + i]);
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)m * ldo] = red[0];
+}
+
 }  // namespace bp8
 }  // namespace eqd
 
@@ -470,6 +775,49 @@ struct eqd_bp8 {
     int node_blocks() const { return (c.nf + 3) / 4; }
     int ldS() const { return (c.nstate + 1) & ~1; }
     int state_blocks() const { return (c.nstate + 255) / 256; }
+};
+
+struct eqd_bp8_ensemble {
+    eqd_bp8* base = nullptr;
+    int M = 0, ldU = 0;
+    std::vector<double> ha, hsig, hQ0;
+    DevMem pa, psig, pQ0, psrc, Vprev, Vm, fail, k, uprev, unew, ya, yb, D, partial, ctl, saves, tfix;
+    int64_t save_cap = 0, nsaved = 0;
+    hipStream_t st = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_steps = 4;
+    Ctl* h_ctl = nullptr;
+    double gemm_flops_per_call = 0;
+    ~eqd_bp8_ensemble() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (h_ctl) (void)hipHostFree(h_ctl);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    int node_blocks() const { return (base->c.nf + 255) / 256; }
+    EnsDev dev() const {
+        EnsDev d;
+        d.c = base->c;
+        d.Lx = base->Lx.as<double>();
+        d.src = base->src.as<double>();
+        d.active = base->active.as<unsigned char>();
+        d.pa = pa.as<double>();
+        d.psig = psig.as<double>();
+        d.pQ0 = pQ0.as<double>();
+        d.psrc = psrc.as<double>();
+        d.M = M;
+        d.ldU = ldU;
+        d.Vprev = Vprev.as<double>();
+        d.Vm = Vm.as<double>();
+        d.fail = fail.as<int>();
+        d.k = k.as<double>();
+        d.uprev = uprev.as<double>();
+        d.unew = unew.as<double>();
+        d.partial = partial.as<double>();
+        d.ctl = ctl.as<Ctl>();
+        d.saves = saves.as<double>();
+        d.nsave_max = save_cap;
+        return d;
+    }
 };
 
 namespace {
@@ -736,6 +1084,202 @@ void integrate(eqd_bp8& m, double t0, double tf, double saveat, double reltol, i
                            hipMemcpyDeviceToHost));
 }
 
+// K S over the members' slip vectors (rows 0 .. 2nf-1 of y, member stride ldY) into E.D.
+void ens_gemm(eqd_bp8_ensemble& E, const double* y, int64_t ldY) {
+    const int n2 = E.base->c.n2;
+    hipLaunchKernelGGL(k_dgemm_mfma, dim3((n2 + kGemmBM - 1) / kGemmBM, (E.M + kGemmBN - 1) / kGemmBN), dim3(256), 0,
+                       E.st, n2, E.M, n2, E.base->KR.as<double>(), (int64_t)n2, y, ldY, E.D.as<double>(), (int64_t)n2);
+}
+
+void ens_stage(eqd_bp8_ensemble& E, int stage, const double* y, int64_t ldY, double* kout, double* ynext) {
+    hipLaunchKernelGGL(k_ens_stage, dim3(E.node_blocks(), E.M), dim3(256), 0, E.st, E.dev(), stage, y, ldY,
+                       E.D.as<double>(), (int64_t)E.base->c.n2, kout, ynext, E.tfix.as<double>());
+}
+
+// rhs! of every member at y (member stride ldY) into out (member stride ldU), at times tfix.
+void ens_rhs(eqd_bp8_ensemble& E, const double* y, int64_t ldY, double* out) {
+    ens_gemm(E, y, ldY);
+    ens_stage(E, 0, y, ldY, out, nullptr);
+    HIPCHK(hipGetLastError());
+}
+
+void ens_check_newton(eqd_bp8_ensemble& E) {
+    std::vector<int> f(E.M);
+    HIPCHK(hipMemcpyAsync(f.data(), E.fail.p, E.M * sizeof(int), hipMemcpyDeviceToHost, E.st));
+    HIPCHK(hipStreamSynchronize(E.st));
+    for (int m = 0; m < E.M; ++m)
+        if (f[m]) {
+            HIPCHK(hipMemset(E.fail.p, 0, E.M * sizeof(int)));
+            throw Error(EQD_ERR_NEWTON, "solve_slip_rate failed to converge (member " + std::to_string(m) + ")");
+        }
+}
+
+void ens_initial_state(const eqd_bp8_ensemble& E, int m, double* u) {
+    const eqd_bp8& B = *E.base;
+    const eqd_bp8_params& p = B.p;
+    std::fill(u, u + B.c.nstate, 0.0);
+    const double a = E.ha[m], sig = E.hsig[m];
+    const double Vm = std::sqrt(p.V_init * p.V_init + p.V_zero * p.V_zero);
+    const double f = (p.tau_init - B.c.eta * Vm) / sig;
+    const double theta0 = (p.D_RS / p.V_star) * std::exp((a * std::log(2 * p.V_star * std::sinh(f / a) / Vm) - p.f_star) / p.b);
+    for (int i = 0; i < B.c.nf; ++i) u[2 * B.c.nf + i] = std::log(theta0);
+}
+
+void ens_integrate(eqd_bp8_ensemble& E, double t0, double tf, double saveat, double reltol_all,
+                   const double* reltols, int64_t maxiters, int64_t nsave_max, eqd_bp8_stats* stats,
+                   double* loop_ms) {
+    if (!(tf > t0) || !(saveat > 0) || nsave_max < 1)
+        throw Error(EQD_ERR_ARG, "need tf > t0, saveat > 0 and nsave_max >= 1");
+    std::vector<double> rtol(E.M, reltol_all);
+    if (reltols) std::copy(reltols, reltols + E.M, rtol.begin());
+    for (double r : rtol)
+        if (!(r > 0)) throw Error(EQD_ERR_ARG, "reltol must be positive");
+    if (maxiters <= 0) maxiters = 100000;
+    const eqd_bp8& B = *E.base;
+    const int M = E.M, ns = B.c.nstate, ld = E.ldU, nf = B.c.nf;
+    if (nsave_max > E.save_cap) {
+        E.saves.alloc((size_t)M * nsave_max * ld * 8);
+        E.save_cap = nsave_max;
+        if (E.exec) {
+            HIPCHK(hipGraphExecDestroy(E.exec));
+            E.exec = nullptr;
+        }
+    }
+    E.nsaved = 0;
+    std::vector<double> u0((size_t)ld * M, 0.0), f0((size_t)ld * M), f1((size_t)ld * M), u1((size_t)ld * M, 0.0);
+    for (int m = 0; m < M; ++m) ens_initial_state(E, m, u0.data() + (size_t)m * ld);
+    std::vector<double> tf0(M, t0);
+    HIPCHK(hipMemcpyAsync(E.tfix.p, tf0.data(), M * 8, hipMemcpyHostToDevice, E.st));
+    HIPCHK(hipMemcpyAsync(E.uprev.p, u0.data(), u0.size() * 8, hipMemcpyHostToDevice, E.st));
+    ens_rhs(E, E.uprev.as<double>(), ld, E.k.as<double>());
+    HIPCHK(hipMemcpyAsync(f0.data(), E.k.p, f0.size() * 8, hipMemcpyDeviceToHost, E.st));
+    ens_check_newton(E);
+    // Hairer's initial step, per member
+    std::vector<double> dt0(M), sk(ns), dtv(M);
+    for (int m = 0; m < M; ++m) {
+        const double reltol = rtol[m];
+        const double* u = u0.data() + (size_t)m * ld;
+        const double* f = f0.data() + (size_t)m * ld;
+        double s0 = 0, s1 = 0;
+        for (int e = 0; e < ns; ++e) {
+            sk[e] = (e < 2 * nf ? 1e-14 : (e < 3 * nf ? 1e-10 : 1e-3)) + std::fabs(u[e]) * reltol;
+            s0 += (u[e] / sk[e]) * (u[e] / sk[e]);
+            s1 += (f[e] / sk[e]) * (f[e] / sk[e]);
+        }
+        const double d0 = std::sqrt(s0 / ns), d1 = std::sqrt(s1 / ns);
+        double d = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * (d0 / d1);
+        dt0[m] = std::min(d, tf - t0);
+        for (int e = 0; e < ns; ++e) u1[(size_t)m * ld + e] = u[e] + dt0[m] * f[e];
+        tf0[m] = t0 + dt0[m];
+    }
+    HIPCHK(hipMemcpyAsync(E.tfix.p, tf0.data(), M * 8, hipMemcpyHostToDevice, E.st));
+    HIPCHK(hipMemcpyAsync(E.ya.p, u1.data(), u1.size() * 8, hipMemcpyHostToDevice, E.st));
+    ens_rhs(E, E.ya.as<double>(), ld, E.yb.as<double>());
+    HIPCHK(hipMemcpyAsync(f1.data(), E.yb.p, f1.size() * 8, hipMemcpyDeviceToHost, E.st));
+    ens_check_newton(E);
+    std::vector<Ctl> hc(M);
+    for (int m = 0; m < M; ++m) {
+        const double reltol = rtol[m];
+        const double* u = u0.data() + (size_t)m * ld;
+        const double* f = f0.data() + (size_t)m * ld;
+        const double* g = f1.data() + (size_t)m * ld;
+        double s2 = 0, s1 = 0;
+        for (int e = 0; e < ns; ++e) {
+            const double skv = (e < 2 * nf ? 1e-14 : (e < 3 * nf ? 1e-10 : 1e-3)) + std::fabs(u[e]) * reltol;
+            s2 += ((g[e] - f[e]) / skv) * ((g[e] - f[e]) / skv);
+            s1 += (f[e] / skv) * (f[e] / skv);
+        }
+        const double d1 = std::sqrt(s1 / ns), d2 = std::sqrt(s2 / ns) / dt0[m];
+        const double md = std::max(d1, d2);
+        const double dt1 = md <= 1e-15 ? std::max(1e-6, dt0[m] * 1e-3) : std::pow(10.0, -(2 + std::log10(md)) / 5.0);
+        const double dt = std::min(std::min(100 * dt0[m], dt1), tf - t0);
+        Ctl& c = hc[m];
+        std::memset(&c, 0, sizeof(Ctl));
+        c.t = t0;
+        c.t0 = t0;
+        c.tf = tf;
+        c.saveat = saveat;
+        c.reltol = reltol;
+        c.qold = 1e-4;
+        c.nsave_max = nsave_max;
+        c.maxiters = maxiters;
+        c.nrhs = 2;
+        c.nsaved = 1;
+        c.tstop = (B.c.t_off > t0 && B.c.t_off < tf) ? B.c.t_off : tf;
+        c.clipped = dt >= c.tstop - t0;
+        c.dt = c.clipped ? c.tstop - t0 : dt;
+    }
+    HIPCHK(hipMemcpyAsync(E.ctl.p, hc.data(), M * sizeof(Ctl), hipMemcpyHostToDevice, E.st));
+    HIPCHK(hipMemcpy2DAsync(E.saves.p, (size_t)E.save_cap * ld * 8, E.uprev.p, (size_t)ld * 8, (size_t)ld * 8, M,
+                            hipMemcpyDeviceToDevice, E.st));
+    const dim3 cgrid((ns + 255) / 256, M);
+    hipLaunchKernelGGL(k_ens_commit, cgrid, dim3(256), 0, E.st, E.dev(), E.ya.as<double>(), 1);
+    HIPCHK(hipGetLastError());
+    if (!E.exec) {
+        double* kk = E.k.as<double>();
+        const size_t kst = (size_t)ld * M;
+        double* ya = E.ya.as<double>();
+        double* yb = E.yb.as<double>();
+        double* un = E.unew.as<double>();
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal));
+        for (int stp = 0; stp < E.graph_steps; ++stp) {
+            const double* in[8] = {nullptr, nullptr, ya, yb, ya, yb, ya, un};
+            double* out[8] = {nullptr, nullptr, yb, ya, yb, ya, un, nullptr};
+            for (int stage = 2; stage <= 7; ++stage) {
+                ens_gemm(E, in[stage], ld);
+                ens_stage(E, stage, in[stage], ld, kk + (size_t)(stage - 1) * kst, out[stage]);
+            }
+            hipLaunchKernelGGL(k_ens_control, dim3((M + 63) / 64), dim3(64), 0, E.st, E.dev(), E.node_blocks());
+            hipLaunchKernelGGL(k_ens_commit, cgrid, dim3(256), 0, E.st, E.dev(), ya, 0);
+        }
+        HIPCHK(hipStreamEndCapture(E.st, &g));
+        HIPCHK(hipGraphInstantiate(&E.exec, g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, E.st));
+    const int64_t max_replays = maxiters / E.graph_steps + 2;
+    for (int64_t rep = 0;; ++rep) {
+        HIPCHK(hipGraphLaunch(E.exec, E.st));
+        HIPCHK(hipMemcpyAsync(E.h_ctl, E.ctl.p, M * sizeof(Ctl), hipMemcpyDeviceToHost, E.st));
+        HIPCHK(hipStreamSynchronize(E.st));
+        bool all = true;
+        for (int m = 0; m < M; ++m) all = all && E.h_ctl[m].done;
+        if (all) break;
+        if (rep > max_replays) throw Error(EQD_ERR_DEVICE, "ensemble step loop did not terminate");
+    }
+    HIPCHK(hipEventRecord(e1, E.st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (loop_ms) *loop_ms = ms;
+    int64_t nsv = E.h_ctl[0].nsaved;
+    int failed = -1;
+    for (int m = 0; m < M; ++m) {
+        const Ctl& r = E.h_ctl[m];
+        nsv = std::min<int64_t>(nsv, r.nsaved);
+        if (r.retcode == 2 && failed < 0) failed = m;
+        if (stats) {
+            stats[m].naccept = r.naccept;
+            stats[m].nreject = r.nreject;
+            stats[m].nrhs = r.nrhs;
+            stats[m].nsaved = r.nsaved;
+            stats[m].retcode = r.retcode;
+            stats[m].ms = ms;
+        }
+    }
+    E.nsaved = nsv;
+    if (failed >= 0) {
+        HIPCHK(hipMemset(E.fail.p, 0, M * sizeof(int)));
+        throw Error(EQD_ERR_NEWTON, "solve_slip_rate failed to converge (member " + std::to_string(failed) + ")");
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -933,6 +1477,144 @@ eqd_status eqd_bp8_saved_vmax(eqd_bp8* m, double* vmax, int64_t n) {
         HIPCHK(hipGetLastError());
         check_newton(*m);
         HIPCHK(hipMemcpy(vmax, out.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+eqd_status eqd_bp8_ensemble_create(eqd_bp8* base, int members, const double* Q0, const double* a,
+                                   const double* sigma0, eqd_bp8_ensemble** out) {
+    return guard([&] {
+        if (!base || !out || members < 1) throw Error(EQD_ERR_ARG, "need a model and at least one member");
+        auto E = std::make_unique<eqd_bp8_ensemble>();
+        E->base = base;
+        E->M = members;
+        E->ldU = (base->c.nstate + 1) & ~1;
+        E->ha.assign(members, base->p.a);
+        E->hsig.assign(members, base->p.sigma0);
+        E->hQ0.assign(members, base->p.Q0);
+        for (int m = 0; m < members; ++m) {
+            if (Q0) E->hQ0[m] = Q0[m];
+            if (a) E->ha[m] = a[m];
+            if (sigma0) E->hsig[m] = sigma0[m];
+            if (!(E->ha[m] > 0) || !(E->hsig[m] > 0)) throw Error(EQD_ERR_ARG, "a and sigma0 must be positive");
+        }
+        const size_t nf = base->c.nf, M = members, ld = E->ldU;
+        HIPCHK(hipStreamCreateWithFlags(&E->st, hipStreamNonBlocking));
+        HIPCHK(hipHostMalloc((void**)&E->h_ctl, M * sizeof(Ctl), hipHostMallocDefault));
+        std::memset(E->h_ctl, 0, M * sizeof(Ctl));
+        E->pa.alloc(M * 8);
+        E->psig.alloc(M * 8);
+        E->pQ0.alloc(M * 8);
+        E->psrc.alloc(M * 8);
+        E->tfix.alloc(M * 8);
+        E->Vprev.alloc(nf * M * 8);
+        E->Vm.alloc(nf * M * 8);
+        E->fail.alloc(M * sizeof(int));
+        E->k.alloc(7 * ld * M * 8);
+        for (DevMem* v : {&E->uprev, &E->unew, &E->ya, &E->yb}) v->alloc(ld * M * 8);
+        E->D.alloc((size_t)base->c.n2 * M * 8);
+        E->partial.alloc((size_t)E->node_blocks() * M * 8);
+        E->ctl.alloc(M * sizeof(Ctl));
+        HIPCHK(hipMemcpy(E->pa.p, E->ha.data(), M * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(E->psig.p, E->hsig.data(), M * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(E->pQ0.p, E->hQ0.data(), M * 8, hipMemcpyHostToDevice));
+        std::vector<double> srate(M);
+        const eqd_bp8_params& bp = base->p;
+        for (size_t m = 0; m < M; ++m) srate[m] = (E->hQ0[m] / bp.L_fwid) / (bp.beta * bp.phi);
+        HIPCHK(hipMemcpy(E->psrc.p, srate.data(), M * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(E->uprev.p, 0, E->uprev.bytes));
+        HIPCHK(hipMemset(E->unew.p, 0, E->unew.bytes));
+        HIPCHK(hipMemset(E->fail.p, 0, M * sizeof(int)));
+        std::vector<double> vp(nf * M, 1e-12);
+        HIPCHK(hipMemcpy(E->Vprev.p, vp.data(), vp.size() * 8, hipMemcpyHostToDevice));
+        E->gemm_flops_per_call = 2.0 * (double)base->c.n2 * base->c.n2 * members;
+        *out = E.release();
+    });
+}
+
+eqd_status eqd_bp8_ensemble_destroy(eqd_bp8_ensemble* e) {
+    return guard([&] { delete e; });
+}
+
+eqd_status eqd_bp8_ensemble_integrate(eqd_bp8_ensemble* e, double t0, double tf, double saveat, double reltol,
+                                      const double* member_reltol, int64_t maxiters, int64_t nsave_max,
+                                      eqd_bp8_stats* stats, double* loop_ms) {
+    return guard([&] {
+        if (!e) throw Error(EQD_ERR_ARG, "null ensemble");
+        ens_integrate(*e, t0, tf, saveat, reltol, member_reltol, maxiters, nsave_max, stats, loop_ms);
+    });
+}
+
+eqd_status eqd_bp8_ensemble_saved(const eqd_bp8_ensemble* e, int member, double* ts, double* us, int64_t* nsaved) {
+    return guard([&] {
+        if (!e) throw Error(EQD_ERR_ARG, "null ensemble");
+        if (member < 0 || member >= e->M) throw Error(EQD_ERR_ARG, "member out of range");
+        const int ns = e->base->c.nstate;
+        const Ctl& c = e->h_ctl[member];
+        const int64_t n = c.nsaved;
+        if (nsaved) *nsaved = n;
+        if (ts) {
+            for (int64_t j = 0; j < n; ++j) ts[j] = c.t0 + (double)j * c.saveat;
+            if (c.end_save && n > 0) ts[n - 1] = c.tf;
+        }
+        if (us && n > 0)
+            HIPCHK(hipMemcpy2D(us, (size_t)ns * 8, e->saves.as<double>() + (size_t)member * e->save_cap * e->ldU,
+                               (size_t)e->ldU * 8, (size_t)ns * 8, (size_t)n, hipMemcpyDeviceToHost));
+    });
+}
+
+eqd_status eqd_bp8_ensemble_saved_vmax(eqd_bp8_ensemble* e, double* vmax, int64_t n, double* gemm_ms) {
+    return guard([&] {
+        if (!e || !vmax) throw Error(EQD_ERR_ARG, "null argument");
+        if (n > e->nsaved) throw Error(EQD_ERR_ARG, "more values requested than states saved");
+        if (n <= 0) return;
+        eqd_bp8_ensemble& E = *e;
+        DevMem out((size_t)E.M * n * 8);
+        std::vector<double> tz(E.M, 0.0);
+        HIPCHK(hipMemcpyAsync(E.tfix.p, tz.data(), E.M * 8, hipMemcpyHostToDevice, E.st));
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, E.st));
+        const int64_t ldY = E.save_cap * E.ldU;
+        for (int64_t j = 0; j < n; ++j) {
+            const double* y = E.saves.as<double>() + (size_t)j * E.ldU;
+            ens_rhs(E, y, ldY, E.yb.as<double>());
+            hipLaunchKernelGGL(k_ens_vmax, dim3(E.M), dim3(256), 0, E.st, E.Vm.as<double>(), E.base->c.nf,
+                               out.as<double>() + j, (int64_t)n);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(e1, E.st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (gemm_ms) *gemm_ms = ms;
+        ens_check_newton(E);
+        HIPCHK(hipMemcpy(vmax, out.p, (size_t)E.M * n * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+/* Time of one K S product over all members (measurement hook): average ms over `reps` GEMMs
+ * on the members' current states, and the flops of one. */
+eqd_status eqd_bp8_ensemble_gemm_timing(eqd_bp8_ensemble* e, int reps, double* ms_per_gemm, double* flops) {
+    return guard([&] {
+        if (!e || reps < 1) throw Error(EQD_ERR_ARG, "null ensemble or reps < 1");
+        eqd_bp8_ensemble& E = *e;
+        ens_gemm(E, E.uprev.as<double>(), E.ldU);
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, E.st));
+        for (int r = 0; r < reps; ++r) ens_gemm(E, E.uprev.as<double>(), E.ldU);
+        HIPCHK(hipEventRecord(e1, E.st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (ms_per_gemm) *ms_per_gemm = ms / reps;
+        if (flops) *flops = E.gemm_flops_per_call;
     });
 }
 

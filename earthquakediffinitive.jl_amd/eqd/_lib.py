@@ -34,6 +34,8 @@ EXPORTED = [
     "eqd_server_create", "eqd_server_destroy", "eqd_server_add_grid", "eqd_server_submit_columns",
     "eqd_server_submit_forward", "eqd_server_step", "eqd_server_completed", "eqd_server_result",
     "eqd_server_grid_info",
+    "eqd_bp8_ensemble_create", "eqd_bp8_ensemble_destroy", "eqd_bp8_ensemble_integrate",
+    "eqd_bp8_ensemble_saved", "eqd_bp8_ensemble_saved_vmax", "eqd_bp8_ensemble_gemm_timing",
 ]
 
 INJECTION_GAUSSIAN, INJECTION_PEACEMAN = 0, 1
@@ -135,6 +137,14 @@ def lib():
                                         vp, vp, C.POINTER(BP8Stats)]
         L.eqd_bp8_saved_device.argtypes = [vp, C.POINTER(vp), i64p, i64p]
         L.eqd_bp8_saved_vmax.argtypes = [vp, dp, C.c_int64]
+        pd = C.POINTER(C.c_double)
+        L.eqd_bp8_ensemble_create.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(vp)]
+        L.eqd_bp8_ensemble_destroy.argtypes = [vp]
+        L.eqd_bp8_ensemble_integrate.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, vp,
+                                                 C.c_int64, C.c_int64, C.POINTER(BP8Stats), pd]
+        L.eqd_bp8_ensemble_saved.argtypes = [vp, C.c_int, vp, vp, C.POINTER(C.c_int64)]
+        L.eqd_bp8_ensemble_saved_vmax.argtypes = [vp, vp, C.c_int64, pd]
+        L.eqd_bp8_ensemble_gemm_timing.argtypes = [vp, C.c_int, pd, pd]
         L.eqd_server_create.argtypes = [C.POINTER(CgOpts), C.c_size_t, C.POINTER(vp)]
         L.eqd_server_destroy.argtypes = [vp]
         L.eqd_server_add_grid.argtypes = [vp, C.POINTER(GridDesc), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]

@@ -228,6 +228,40 @@ eqd_status eqd_bp8_saved_device(const eqd_bp8* m, const double** us_dev, int64_t
 eqd_status eqd_bp8_saved_vmax(eqd_bp8* m, double* vmax, int64_t n);
 
 /* ------------------------------------------------------------------------------
+ * Trajectory ensemble: M copies of run_bp8 (src/BP8.jl:442-470) on one model's K, A_p and
+ * source, each member with its own Q0, a, sigma0 (and reltol), its own Tsit5 step
+ * controller and saves. Members advance in lockstep stages; each stage's K S over all
+ * members is one FP64 MFMA GEMM (K resident, S = the members' slip vectors), followed by
+ * the per-(node, member) Newton in ln V, aging law and A_p p. A member's trajectory is the
+ * one run_bp8 gives for build_model with that member's parameters (no counterpart in the
+ * reference, which runs one trajectory per call).
+ * ------------------------------------------------------------------------------ */
+typedef struct eqd_bp8_ensemble eqd_bp8_ensemble;
+
+/* Q0, a, sigma0: arrays of `members` values (NULL = the model's value for every member).
+ * The model must outlive the ensemble. */
+eqd_status eqd_bp8_ensemble_create(eqd_bp8* model, int members, const double* Q0, const double* a,
+                                   const double* sigma0, eqd_bp8_ensemble** out);
+eqd_status eqd_bp8_ensemble_destroy(eqd_bp8_ensemble* e);
+/* Integrates every member over (t0, tf) as eqd_bp8_integrate does, with reltol for all or
+ * member_reltol[members] (NULL = reltol). stats[members] (may be NULL); retcode 2 marks a
+ * member whose Newton solve failed (the call then returns EQD_ERR_NEWTON). loop_ms: device
+ * time of the step loop (may be NULL). */
+eqd_status eqd_bp8_ensemble_integrate(eqd_bp8_ensemble* e, double t0, double tf, double saveat, double reltol,
+                                      const double* member_reltol, int64_t maxiters, int64_t nsave_max,
+                                      eqd_bp8_stats* stats, double* loop_ms);
+/* A member's save times ts[nsaved] and states us[nsaved * state_len] (host; either may be NULL). */
+eqd_status eqd_bp8_ensemble_saved(const eqd_bp8_ensemble* e, int member, double* ts, double* us,
+                                  int64_t* nsaved);
+/* max_i |V_i| at the first n saves of every member (vmax[member * n + j]), each save
+ * evaluated in save order as eqd_bp8_saved_vmax does; one GEMM per save over all members.
+ * ms: device time (may be NULL). */
+eqd_status eqd_bp8_ensemble_saved_vmax(eqd_bp8_ensemble* e, double* vmax, int64_t n, double* ms);
+/* Measurement hook: average device time of one K S GEMM over all members' current states
+ * (reps launches) and its flop count 2 (2nf)^2 members. */
+eqd_status eqd_bp8_ensemble_gemm_timing(eqd_bp8_ensemble* e, int reps, double* ms_per_gemm, double* flops);
+
+/* ------------------------------------------------------------------------------
  * Serving loop (no counterpart in the reference, whose K build is a batch job): stiffness
  * column and forward-solve requests on several grids, admitted into resident 64-column
  * tile pools as lanes free up (continuous batching), each column's result independent of
