@@ -424,60 +424,112 @@ __global__ void k_reset_cache(Dev d) {
 
 typedef double dvec4 __attribute__((ext_vector_type(4)));
 
-// D[i + ldD m] = sum_k A[i lda + k] B[k + ldB m] (A row-major, B and D column-major), FP64,
-// v_mfma_f64_16x16x4f64: lane l feeds A[l % 16][l / 16] and B[l / 16][l % 16] and holds
-// C[4 j + l / 16][l % 16] in element j. Block tile 64 x 64 (4 waves of 32 x 32), k chunks of
-// 16 staged in LDS.
-constexpr int kGemmBM = 64, kGemmBN = 64, kGemmBK = 16;
-__global__ __launch_bounds__(256) void k_dgemm_mfma(int nr, int nm, int nk, const double* __restrict__ A,
-                                                    int64_t lda, const double* __restrict__ B, int64_t ldB,
-                                                    double* __restrict__ D, int64_t ldD) {
-    __shared__ double As[kGemmBK][kGemmBM + 2];
-    __shared__ double Bs[kGemmBK][kGemmBN + 2];
-    const int i0 = blockIdx.x * kGemmBM, m0 = blockIdx.y * kGemmBN;
-    const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
-    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
-    dvec4 acc[2][2];
+// Split-K partial products P_z = A[:, kz] B[kz, :] (A = K column-major with leading
+// dimension lda, B = the members' slip vectors column-major with leading dimension ldB;
+// z = blockIdx.z covers k in [z kchunk, (z + 1) kchunk)), written to P + z ldP nm (column-major,
+// leading dimension ldP). FP64 v_mfma_f64_16x16x4f64: lane l feeds A[l % 16][k] and B[k][l % 16]
+// and holds C[4 j + l / 16][l % 16] in element j. Block tile kGemmBM x kGemmBN over 4 waves
+// (2 x 2, each 64 x 32), k chunks of kGemmBK double-buffered in LDS (one barrier per chunk, the
+// next chunk's global loads in flight during the MFMAs). Within a chunk, lane group g = l / 16
+// takes k = g kGemmBK / 4 + s in MFMA step s, so a lane's B operands for the chunk are
+// contiguous (ds_read_b128). A tile rows are contiguous in HBM (column-major K): BK runs of
+// BM doubles per chunk.
+constexpr int kGemmBM = 128, kGemmBN = 64, kGemmBK = 16;
+__global__ __launch_bounds__(256) void k_ks_gemm(int nr, int nm, int nk, const double* __restrict__ A, int64_t lda,
+                                                 const double* __restrict__ B, int64_t ldB, double* __restrict__ P,
+                                                 int64_t ldP, int kchunk) {
+    constexpr int BM = kGemmBM, BN = kGemmBN, BK = kGemmBK;
+    constexpr int TM = BM / 2 / 16, TN = BN / 2 / 16;
+    constexpr int LDA_S = BM + 4, LDK = BK + 2;
+    constexpr int AP = BM / 2, ACS = 256 / AP, LA = BK / ACS;  // A: row pairs x columns
+    constexpr int KP = BK / 2, RS = 256 / KP, LB = BN / RS;     // B: k pairs x members
+    constexpr int KS = BK / 4;
+    __shared__ __attribute__((aligned(16))) double As[2][BK * LDA_S];
+    __shared__ __attribute__((aligned(16))) double Bs[2][BN * LDK];
+    const int i0 = blockIdx.x * BM, m0 = blockIdx.y * BN;
+    const int kbeg = blockIdx.z * kchunk, kend = min(nk, kbeg + kchunk);
+    const int t = threadIdx.x, wv = t >> 6, lane = t & 63, g = lane >> 4, r16 = lane & 15;
+    const int wr = (wv >> 1) * (BM / 2), wc = (wv & 1) * (BN / 2);
+    dvec4 acc[TM][TN];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < TM; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < nk; k0 += kGemmBK) {
+        for (int b = 0; b < TN; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
+    const int ap_t = t % AP, ac_t = t / AP;
+    const int kp_t = t % KP, r_t = t / KP;
+    double2 ra[LA], rb[LB];
+    auto gload = [&](int k0) {
+        const int gi = i0 + 2 * ap_t;
 #pragma unroll
-        for (int q = t; q < kGemmBM * kGemmBK; q += 256) {
-            const int r = q / kGemmBK, kk = q % kGemmBK;
-            const int gi = i0 + r, gk = k0 + kk;
-            As[kk][r] = (gi < nr && gk < nk) ? A[(size_t)gi * lda + gk] : 0.0;
-        }
-#pragma unroll
-        for (int q = t; q < kGemmBN * kGemmBK; q += 256) {
-            const int c = q / kGemmBK, kk = q % kGemmBK;
-            const int gm = m0 + c, gk = k0 + kk;
-            Bs[kk][c] = (gm < nm && gk < nk) ? B[(size_t)gk + (size_t)ldB * gm] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < kGemmBK; kk += 4) {
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                const double av = As[kk + lane / 16][wr + a * 16 + lane % 16];
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    const double bv = Bs[kk + lane / 16][wc + b * 16 + lane % 16];
-                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[a][b], 0, 0, 0);
-                }
+        for (int j = 0; j < LA; ++j) {
+            const int gk = k0 + ac_t + ACS * j;
+            double2 v{0.0, 0.0};
+            if (gk < kend) {
+                const double* p = A + (size_t)gk * lda + gi;
+                if (gi + 1 < nr) v = *reinterpret_cast<const double2*>(p);
+                else if (gi < nr) v.x = p[0];
             }
+            ra[j] = v;
         }
+        const int gk = k0 + 2 * kp_t;
+#pragma unroll
+        for (int j = 0; j < LB; ++j) {
+            const int gm = m0 + r_t + RS * j;
+            double2 v{0.0, 0.0};
+            if (gm < nm) {
+                const double* p = B + (size_t)gk + (size_t)ldB * gm;
+                if (gk + 1 < kend) v = *reinterpret_cast<const double2*>(p);
+                else if (gk < kend) v.x = p[0];
+            }
+            rb[j] = v;
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < LA; ++j) *reinterpret_cast<double2*>(&As[buf][(ac_t + ACS * j) * LDA_S + 2 * ap_t]) = ra[j];
+#pragma unroll
+        for (int j = 0; j < LB; ++j) *reinterpret_cast<double2*>(&Bs[buf][(r_t + RS * j) * LDK + 2 * kp_t]) = rb[j];
+    };
+    gload(kbeg);
+    lstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        const bool more = k0 + BK < kend;
+        if (more) gload(k0 + BK);
+        double av[TM][KS], bv[TN][KS];
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int s = 0; s < KS; ++s) av[a][s] = As[buf][(g * KS + s) * LDA_S + wr + a * 16 + r16];
+#pragma unroll
+        for (int b = 0; b < TN; ++b)
+#pragma unroll
+            for (int s = 0; s < KS; s += 2) {
+                const double2 v = *reinterpret_cast<const double2*>(&Bs[buf][(wc + b * 16 + r16) * LDK + g * KS + s]);
+                bv[b][s] = v.x;
+                bv[b][s + 1] = v.y;
+            }
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+#pragma unroll
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int b = 0; b < TN; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a][s], bv[b][s], acc[a][b], 0, 0, 0);
+        if (more) lstore(buf ^ 1);
         __syncthreads();
+        buf ^= 1;
     }
+    double* Pz = P + (size_t)blockIdx.z * ldP * nm;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < TM; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < TN; ++b)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int row = i0 + wr + a * 16 + 4 * j + lane / 16, col = m0 + wc + b * 16 + lane % 16;
-                if (row < nr && col < nm) D[(size_t)row + (size_t)ldD * col] = acc[a][b][j];
+                const int row = i0 + wr + a * 16 + 4 * j + g, col = m0 + wc + b * 16 + r16;
+                if (row < nr && col < nm) Pz[(size_t)row + (size_t)ldP * col] = acc[a][b][j];
             }
 }
 
@@ -501,12 +553,13 @@ struct EnsDev {
     Ctl* ctl;              // M
     double* saves;         // (M x nsave_max) x ldU
     int64_t nsave_max;
+    const double* D;       // splits x 2nf x M: partial K y of the current stage
+    int splits;
 };
 
-// One rhs! per (node, member): thread i over nodes, member m = blockIdx.y. D holds K y
-// (rows 0 .. 2nf-1 of each member column, leading dimension ldD).
+// One rhs! per (node, member): thread i over nodes, member m = blockIdx.y. d.D holds the
+// split-K partials of K y (splits x 2nf x M).
 __global__ __launch_bounds__(256) void k_ens_stage(EnsDev d, int stage, const double* __restrict__ y, int64_t ldY,
-                                                   const double* __restrict__ Dk, int64_t ldD,
                                                    double* __restrict__ kout, double* __restrict__ ynext,
                                                    const double* __restrict__ t_fixed) {
     const Consts& c = d.c;
@@ -521,7 +574,14 @@ __global__ __launch_bounds__(256) void k_ens_stage(EnsDev d, int stage, const do
         const size_t um = (size_t)m * d.ldU;
         const double* ym = y + (size_t)m * ldY;
         const double a = d.pa[m], sigma0 = d.psig[m];
-        const double dtau2 = Dk[(size_t)i + (size_t)ldD * m], dtau3 = Dk[(size_t)c.nf + i + (size_t)ldD * m];
+        // K y: the split-K partials summed in a fixed order
+        const double* Dm = d.D + (size_t)c.n2 * m;
+        const size_t zs = (size_t)c.n2 * d.M;
+        double dtau2 = Dm[i], dtau3 = Dm[c.nf + i];
+        for (int z = 1; z < d.splits; ++z) {
+            dtau2 += Dm[z * zs + i];
+            dtau3 += Dm[z * zs + c.nf + i];
+        }
         const int n = c.n2f, ia = i % n, ib = i / n;
         const double* P = ym + 3 * c.nf;
         double ap = 0.0;
@@ -781,13 +841,14 @@ struct eqd_bp8_ensemble {
     eqd_bp8* base = nullptr;
     int M = 0, ldU = 0;
     std::vector<double> ha, hsig, hQ0;
-    DevMem pa, psig, pQ0, psrc, Vprev, Vm, fail, k, uprev, unew, ya, yb, D, partial, ctl, saves, tfix;
+    DevMem Kc, pa, psig, pQ0, psrc, Vprev, Vm, fail, k, uprev, unew, ya, yb, D, partial, ctl, saves, tfix;
     int64_t save_cap = 0, nsaved = 0;
     hipStream_t st = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_steps = 4;
     Ctl* h_ctl = nullptr;
     double gemm_flops_per_call = 0;
+    int splits = 1, kchunk = 0;  // split-K of the K S GEMM
     ~eqd_bp8_ensemble() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (h_ctl) (void)hipHostFree(h_ctl);
@@ -816,6 +877,8 @@ struct eqd_bp8_ensemble {
         d.ctl = ctl.as<Ctl>();
         d.saves = saves.as<double>();
         d.nsave_max = save_cap;
+        d.D = D.as<double>();
+        d.splits = splits;
         return d;
     }
 };
@@ -1087,13 +1150,14 @@ void integrate(eqd_bp8& m, double t0, double tf, double saveat, double reltol, i
 // K S over the members' slip vectors (rows 0 .. 2nf-1 of y, member stride ldY) into E.D.
 void ens_gemm(eqd_bp8_ensemble& E, const double* y, int64_t ldY) {
     const int n2 = E.base->c.n2;
-    hipLaunchKernelGGL(k_dgemm_mfma, dim3((n2 + kGemmBM - 1) / kGemmBM, (E.M + kGemmBN - 1) / kGemmBN), dim3(256), 0,
-                       E.st, n2, E.M, n2, E.base->KR.as<double>(), (int64_t)n2, y, ldY, E.D.as<double>(), (int64_t)n2);
+    hipLaunchKernelGGL(k_ks_gemm, dim3((n2 + kGemmBM - 1) / kGemmBM, (E.M + kGemmBN - 1) / kGemmBN, E.splits),
+                       dim3(256), 0, E.st, n2, E.M, n2, E.Kc.as<double>(), (int64_t)n2, y, ldY, E.D.as<double>(),
+                       (int64_t)n2, E.kchunk);
 }
 
 void ens_stage(eqd_bp8_ensemble& E, int stage, const double* y, int64_t ldY, double* kout, double* ynext) {
-    hipLaunchKernelGGL(k_ens_stage, dim3(E.node_blocks(), E.M), dim3(256), 0, E.st, E.dev(), stage, y, ldY,
-                       E.D.as<double>(), (int64_t)E.base->c.n2, kout, ynext, E.tfix.as<double>());
+    hipLaunchKernelGGL(k_ens_stage, dim3(E.node_blocks(), E.M), dim3(256), 0, E.st, E.dev(), stage, y, ldY, kout,
+                       ynext, E.tfix.as<double>());
 }
 
 // rhs! of every member at y (member stride ldY) into out (member stride ldU), at times tfix.
@@ -1511,7 +1575,20 @@ eqd_status eqd_bp8_ensemble_create(eqd_bp8* base, int members, const double* Q0,
         E->fail.alloc(M * sizeof(int));
         E->k.alloc(7 * ld * M * 8);
         for (DevMem* v : {&E->uprev, &E->unew, &E->ya, &E->yb}) v->alloc(ld * M * 8);
-        E->D.alloc((size_t)base->c.n2 * M * 8);
+        // K column-major (the model keeps it row-major for its GEMV); split-K so that the GEMM
+        // has about 12 blocks per CU (at most 8 splits), with at least 64 k per split
+        const int n2 = base->c.n2;
+        E->Kc.alloc((size_t)n2 * n2 * 8);
+        hipLaunchKernelGGL(k_transpose, dim3((n2 + 31) / 32, (n2 + 31) / 32), dim3(32, 8), 0, E->st,
+                           base->KR.as<double>(), (long long)n2, n2, E->Kc.as<double>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(E->st));
+        const int tiles = ((n2 + kGemmBM - 1) / kGemmBM) * ((members + kGemmBN - 1) / kGemmBN);
+        int want = std::max(1, std::min(8, (3072 + tiles - 1) / tiles));
+        want = std::max(1, std::min(want, n2 / 64));
+        E->kchunk = ((n2 + want - 1) / want + kGemmBK - 1) / kGemmBK * kGemmBK;
+        E->splits = (n2 + E->kchunk - 1) / E->kchunk;
+        E->D.alloc((size_t)E->splits * n2 * M * 8);
         E->partial.alloc((size_t)E->node_blocks() * M * 8);
         E->ctl.alloc(M * sizeof(Ctl));
         HIPCHK(hipMemcpy(E->pa.p, E->ha.data(), M * 8, hipMemcpyHostToDevice));
