@@ -23,6 +23,11 @@
 #ifndef EQD_PENCIL_WAVES
 #define EQD_PENCIL_WAVES 4
 #endif
+// q is streamed to HBM with non-temporal stores: the update kernel reads it once, and the
+// caches are left to the neighbour lines of p.
+#ifndef EQD_NT_Q
+#define EQD_NT_Q 1
+#endif
 
 namespace eqd {
 
@@ -350,7 +355,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(F ? EQD_
                     for (int c = 0; c < 3; ++c) {
                         const T v = hw * o[c][i];
                         if (MODE == 0) {
+#if EQD_NT_Q
+                            __builtin_nontemporal_store(v, q + e + c * 64);
+#else
                             q[e + c * 64] = v;
+#endif
                             d += (double)in[e + c * 64] * (double)v;
                         } else {
                             const double rv = (double)b[e + c * 64] - (double)v;
