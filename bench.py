@@ -127,6 +127,89 @@ def bench_bp8(args):
     return res
 
 
+def bench_production(args):
+    """Production-size stiffness build: dz = 25 m, L_fault = 1600, L_normal = 800, l_f = 1600
+    (3,294,918 DOF, 16641 fault nodes, 33282 columns). The :toeplitz build (10 solves, K filled
+    on the device, 8.9 GB) and a seeded 512-column subset of the :exact build (mixed precision);
+    the Toeplitz columns are checked against the exact ones (< 1e-3 relative, the reference's own
+    Toeplitz guard scale), and mixed against batched FP64 on 8 of the columns (1e-8)."""
+    mu, nu = 32.04e9, 0.25
+    lam = 2 * mu * nu / (1 - 2 * nu)
+    g = eqd.bp8_grid_pair(25.0, 1600.0, 800.0)
+    fe = eqd.FaultElasticity(g, lam, mu, l_f=1600.0, precision="mixed", tile_k=64)
+    nf2 = 2 * eqd.frictional_node_count(fe)
+    rng = np.random.default_rng(20261203)
+    cols = np.sort(rng.choice(nf2, size=args.prod_cols, replace=False)).astype(np.int32)
+    res = {"dof": fe.Ntot, "columns_total": nf2, "exact_subset": len(cols)}
+    Kt = DeviceBuffer(nf2 * nf2 * 8)
+    t0 = time.perf_counter()
+    _lib.check(_lib.lib().eqd_fault_stiffness_device(fe.rs.handle, _lib.STIFFNESS_TOEPLITZ, None, 0, Kt.ptr, nf2))
+    res["toeplitz_build_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+    # the Toeplitz K's columns of the subset (device -> host, column by column)
+    Ktc = np.zeros((nf2, len(cols)), order="F")
+    for j, c in enumerate(cols):
+        src = ctypes.c_void_p(Kt.ptr.value + int(c) * nf2 * 8)
+        rc = Kt.hip.hipMemcpy(ctypes.c_void_p(Ktc[:, j].ctypes.data), src, ctypes.c_size_t(nf2 * 8), 2)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed: {rc}")
+    Kt.free()
+    Ke = DeviceBuffer(nf2 * len(cols) * 8)
+    t0 = time.perf_counter()
+    _lib.check(_lib.lib().eqd_fault_stiffness_device(fe.rs.handle, _lib.STIFFNESS_EXACT, cols.ctypes.data, len(cols),
+                                                     Ke.ptr, nf2))
+    te = time.perf_counter() - t0
+    K_ex = Ke.to_host((nf2, len(cols)))
+    Ke.free()
+    rep = eqd.solver_report(fe.rs)
+    res["exact_subset_ms"] = round(te * 1e3, 1)
+    res["exact_K_columns_per_s"] = round(len(cols) / te, 2)
+    res["exact_mean_cg_iterations"] = round(rep["iterations"] / max(1, rep["solves"]), 1)
+    res["exact_unconverged"] = rep["unconverged"]
+    col_err = np.linalg.norm(Ktc - K_ex, axis=0) / np.linalg.norm(K_ex, axis=0)
+    res["toeplitz_vs_exact_max_col_rel_err"] = float(col_err.max())
+    res["toeplitz_vs_exact_median_col_rel_err"] = float(np.median(col_err))
+    res["toeplitz_cols_within_1e-3"] = round(float(np.mean(col_err < 1e-3)), 4)
+    res["toeplitz_vs_exact_frobenius_rel_err"] = float(np.linalg.norm(Ktc - K_ex) / np.linalg.norm(K_ex))
+    fe64 = eqd.FaultElasticity(g, lam, mu, l_f=1600.0, precision="fp64", tile_k=8)
+    sub = cols[:8]
+    K64 = eqd.fault_stiffness(fe64, cols=sub)
+    res["mixed_vs_fp64_rel_err_8cols"] = float(np.linalg.norm(K_ex[:, :8] - K64) / np.linalg.norm(K64))
+    print(f"bench: dz=25 production: toeplitz {res['toeplitz_build_ms']} ms, exact {len(cols)} cols in "
+          f"{res['exact_subset_ms']} ms, toeplitz vs exact max col err {col_err.max():.2e}", file=sys.stderr, flush=True)
+    return res
+
+
+def bench_ensemble(args):
+    """Trajectory ensemble on MFMA: dz = 20 m, L_fault = 800, L_normal = 400, l_f = 800 (6561 fault
+    nodes, K 13122 x 13122 FP64 = 1.38 GB from the device :toeplitz build), 256 members with seeded
+    perturbations of Q0 (+-20%), a - b (+-10%) and sigma0 (+-5%), Gaussian source, integrated
+    over args.ens_hours with saveat 3600 s. K S per stage is one FP64 MFMA GEMM (2 * 13122^2 *
+    256 = 88.2 GFLOP); its roofline is the MI355X dense FP64 matrix peak."""
+    from eqd import bp8
+    from eqd.ensemble import Ensemble, perturbed_members
+    par = bp8.benchmark_parameters(l_f=800.0, dz=20.0)
+    t0 = time.perf_counter()
+    m = bp8.build_model(par=par, dz=20.0, L_fault=800.0, L_normal=400.0, stiffness="toeplitz", precision="mixed")
+    t_build = time.perf_counter() - t0
+    mem = perturbed_members(m.par, args.ens_members, seed=20261203)
+    e = Ensemble(m, args.ens_members, **mem)
+    t0 = time.perf_counter()
+    st = e.run(tspan=(0.0, args.ens_hours * 3600.0), saveat=3600.0)
+    t_run = time.perf_counter() - t0
+    ms_gemm, flops = e.gemm_timing(20)
+    tf = flops / (ms_gemm * 1e-3)
+    acc = [s.naccept for s in st]
+    res = {"nf": m.nf, "members": args.ens_members, "hours": args.ens_hours, "K_build_ms": round(t_build * 1e3, 1),
+           "run_ms": round(t_run * 1e3, 1), "device_loop_ms": round(e.loop_ms, 1),
+           "accepted_steps_min_max": [int(min(acc)), int(max(acc))],
+           "member_steps_per_s": round(sum(acc) / (e.loop_ms * 1e-3), 1),
+           "gemm": {"bound": "mfma", "achieved": round(tf / 1e12, 2), "peak": 78.6, "unit": "TFLOP/s",
+                    "frac": round(tf / 78.6e12, 4), "ms_per_gemm": round(ms_gemm, 3), "flop_per_gemm": flops}}
+    print(f"bench: ensemble dz=20 {args.ens_members} members {args.ens_hours} h: K build {t_build * 1e3:.0f} ms, "
+          f"loop {e.loop_ms:.0f} ms, GEMM {ms_gemm:.3f} ms = {tf / 1e12:.1f} TFLOP/s", file=sys.stderr, flush=True)
+    return res
+
+
 def bench_serving(args):
     """Serving loop under a seeded Poisson stream (mean args.serve_rate req/s for
     args.serve_seconds s) over three grids with l_f = L_fault: (dz, L_fault, L_normal) =
@@ -223,6 +306,11 @@ def main():
     ap.add_argument("--no_extras", action="store_true", help="only the timed mixed-precision steps")
     ap.add_argument("--no_bp8", action="store_true", help="skip the BP8 time-loop measurement")
     ap.add_argument("--no_serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--no_production", action="store_true", help="skip the dz = 25 m production-size build")
+    ap.add_argument("--prod_cols", type=int, default=512, help="exact columns of the dz = 25 m subset")
+    ap.add_argument("--no_ensemble", action="store_true", help="skip the dz = 20 m trajectory ensemble")
+    ap.add_argument("--ens_members", type=int, default=256)
+    ap.add_argument("--ens_hours", type=float, default=100.0)
     ap.add_argument("--serve_seconds", type=float, default=10.0)
     ap.add_argument("--serve_rate", type=float, default=10.0,
                     help="mean request rate; 50 req/s over this grid mix exceeds one GPU (see DESIGN.md)")
@@ -354,6 +442,10 @@ def main():
     Kbuf.free()
     if not args.no_extras and not args.no_bp8:
         out["bp8_30day_gs"] = bench_bp8(args)
+    if not args.no_extras and not args.no_production:
+        out["production_dz25"] = bench_production(args)
+    if not args.no_extras and not args.no_ensemble:
+        out["ensemble_dz20"] = bench_ensemble(args)
     if not args.no_extras and not args.no_serve:
         out["serving"] = bench_serving(args)
     print(json.dumps(out), flush=True)

@@ -98,6 +98,7 @@ struct eqd_op {
     }
     PencilView<double> p64{};
     PencilView<float> p32{};
+    int G = kGroups;  // workgroups per tile of the node-parallel kernels (set at creation)
     int64_t lin(int i1, int i2, int i3) const { return i1 + (int64_t)g.n1 * (i2 + (int64_t)g.n23 * i3); }
 };
 
@@ -281,7 +282,7 @@ void ensure_ws(eqd_solver* s, int K, int ntiles, int prec) {
     w->q.alloc(w->nvec * t);
     if (prec == EQD_PREC_MIXED) w->xc.alloc(w->nvec * sizeof(float));
     w->fields.alloc(slots * 2 * op.g.n23 * op.g.n23 * 8);
-    w->partial.alloc((size_t)ntiles * kGroups * K * 8);
+    w->partial.alloc((size_t)ntiles * op.G * K * 8);
     w->cs.alloc(slots * sizeof(ColState));
     w->tile_on.alloc(ntiles * sizeof(int));
     w->tile_rel.alloc(ntiles * sizeof(int));
@@ -299,7 +300,7 @@ struct Engine {
         const eqd_op& op = *s->op;
         const bool mixed = std::is_same<T, float>::value;
         const int R = s->o.reliable_interval > 0 ? s->o.reliable_interval : 32;
-        dim3 grid(kGroups, w.ntiles);
+        dim3 grid(s->op->G, w.ntiles);
         T* r = w.r.as<T>();
         T* p = w.p.as<T>();
         T* q = w.q.as<T>();
@@ -329,7 +330,7 @@ struct Engine {
             }
         });
         timed(3, [&] {
-            hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, 0, (int)mixed, R, cs, part, on, rel,
+            hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, op.G, 0, (int)mixed, R, cs, part, on, rel,
                                nullptr);
         });
         timed(1, [&] {
@@ -344,7 +345,7 @@ struct Engine {
                                    r, p, q, cs, on, part);
         });
         timed(3, [&] {
-            hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, 1, (int)mixed, R, cs, part, on, rel,
+            hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, op.G, 1, (int)mixed, R, cs, part, on, rel,
                                nullptr);
         });
         timed(2, [&] {
@@ -380,7 +381,7 @@ struct Engine {
                                        w.b.as<double>(), r, cs, rel, part);
             });
             timed(3, [&] {
-                hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, 2, 1, R, cs, part, on, rel,
+                hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, op.G, 2, 1, R, cs, part, on, rel,
                                    nullptr);
             });
         }
@@ -391,11 +392,11 @@ struct Engine {
     static void init_state(eqd_solver* s, hipStream_t st, const int* mask = nullptr) {
         Workspace& w = *s->ws;
         const eqd_op& op = *s->op;
-        dim3 grid(kGroups, w.ntiles);
+        dim3 grid(s->op->G, w.ntiles);
         hipLaunchKernelGGL((k_selfdot<double, K>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface,
                            w.b.as<double>(), w.partial.as<double>());
         const int64_t itmax = s->o.itmax > 0 ? s->o.itmax : 2 * op.ntot;
-        hipLaunchKernelGGL((k_init_state<K>), dim3(w.ntiles), dim3(K), 0, st, w.cs.as<ColState>(),
+        hipLaunchKernelGGL((k_init_state<K>), dim3(w.ntiles), dim3(K), 0, st, op.G, w.cs.as<ColState>(),
                            w.partial.as<double>(), w.colid.as<int64_t>(), s->o.rtol, s->o.atol, itmax,
                            w.tile_on.as<int>(), mask);
         HIPCHK(hipGetLastError());
@@ -425,7 +426,7 @@ struct Engine {
             tile_iters += act;
         }
         if (std::is_same<T, float>::value) {
-            dim3 grid(kGroups, w.ntiles);
+            dim3 grid(s->op->G, w.ntiles);
             hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, s->op->NN, w.x.as<double>(),
                                w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1, (const int*)nullptr);
             HIPCHK(hipGetLastError());
@@ -503,7 +504,7 @@ struct Engine {
             tile_iters += (int64_t)act * gi;
         }
         if (std::is_same<T, float>::value) {  // fold any correction still held in FP32
-            dim3 grid(kGroups, w.ntiles);
+            dim3 grid(s->op->G, w.ntiles);
             hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, s->op->NN, w.x.as<double>(),
                                w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1, (const int*)nullptr);
             HIPCHK(hipGetLastError());
@@ -526,8 +527,8 @@ struct Engine {
         Workspace& w = *s->ws;
         const eqd_op& op = *s->op;
         hipStream_t st = s->stream;
-        dim3 grid(kGroups, 1);
-        std::vector<double> part(kGroups);
+        dim3 grid(op.G, 1);
+        std::vector<double> part(op.G);
         ColState h;
         // the state was initialised on the solver's (non-blocking) stream
         HIPCHK(hipMemcpyAsync(&h, w.cs.p, sizeof(ColState), hipMemcpyDeviceToHost, st));
@@ -541,7 +542,7 @@ struct Engine {
         int64_t launches = 0;
         HIPCHK(hipMemcpy(w.tile_on.p, &one, sizeof(int), hipMemcpyHostToDevice));
         auto sum = [&]() {
-            HIPCHK(hipMemcpyAsync(part.data(), w.partial.p, kGroups * 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(part.data(), w.partial.p, op.G * 8, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             double acc = 0.0;
             for (double v : part) acc += v;
@@ -596,7 +597,7 @@ struct Engine {
     static void rhs_from_fields(eqd_solver* s, const int* mask = nullptr) {
         Workspace& w = *s->ws;
         const eqd_op& op = *s->op;
-        dim3 grid(kGroups, w.ntiles);
+        dim3 grid(s->op->G, w.ntiles);
         hipLaunchKernelGGL((k_rhs<T, K>), grid, dim3(kBlock), 0, s->stream, op.rv, op.NN, w.fields.as<double>(),
                            w.ntiles * K, w.b.as<double>(), w.x.as<double>(), w.r.as<T>(), w.p.as<T>(),
                            std::is_same<T, float>::value ? w.xc.as<float>() : nullptr, mask);
@@ -607,7 +608,7 @@ struct Engine {
     static void fold(eqd_solver* s, const int* mask) {
         if constexpr (std::is_same<T, float>::value) {
             Workspace& w = *s->ws;
-            dim3 grid(kGroups, w.ntiles);
+            dim3 grid(s->op->G, w.ntiles);
             hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, s->stream, s->op->NN, w.x.as<double>(),
                                w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1, mask);
             HIPCHK(hipGetLastError());
@@ -1125,6 +1126,15 @@ eqd_status eqd_operator_create(const eqd_grid_desc* g, eqd_op** out) {
         op->v32 = op->view<float>();
         op->p64 = op->pview<double>();
         op->p32 = op->pview<float>();
+        // Groups per tile: about 16 FP32 pencils per workgroup (4 per wave), between 64 and
+        // 1024, so that one tile of a large grid fills the device (dz = 25 m: 64,516 pencils ->
+        // 1024 groups) while small grids keep 64. Depends on the grid only, never on the batch.
+        {
+            const int np = op->p32.NY * op->p32.NY * op->p32.nseg;
+            int G = kGroups;
+            while (G < kMaxGroups && G * 16 < np) G *= 2;
+            op->G = G;
+        }
         op->rv = RhsView{op->tb.NX, op->tb.NY, g->n23, lo, cnt, op->nf, op->up(op->tb.Jt), op->up(op->tb.Jn),
                          op->v64.Hx, op->v64.Hy, op->tab<double>(op->tb.D1full)};
         op->ev = ExtractView{op->tb.NX, op->tb.NY, op->tb.mface, g->n23, lo, cnt, op->nf, g->mu, op->tb.Jext,

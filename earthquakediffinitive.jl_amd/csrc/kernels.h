@@ -1,17 +1,19 @@
 // Device kernels of libeqd (gfx950). Included once, by eqd.hip.
 //
 // Batched layout: see eqd_internal.h. Grid convention for the node-parallel kernels:
-// blockIdx.y = tile, blockIdx.x = group g in [0, kGroups); a block of 256 threads is
-// (256/K) node slots x K columns and walks the tile's nodes with stride
-// kGroups*(256/K). Per-column partial sums are reduced in a fixed order (thread ->
-// block slot 0 -> group index), so a column's arithmetic does not depend on which
-// tile or lane holds it, nor on how many tiles are in flight.
+// blockIdx.y = tile, blockIdx.x = group g in [0, G) with G = gridDim.x groups per tile; a
+// block of 256 threads is (256/K) node slots x K columns and walks the tile's nodes with
+// stride G*(256/K). G is a property of the grid (more groups for larger grids, so one tile
+// alone still fills the device), never of the batch. Per-column partial sums are reduced
+// in a fixed order (thread -> block slot 0 -> group index), so a column's arithmetic does
+// not depend on which tile or lane holds it, nor on how many tiles are in flight.
 #pragma once
 #include "eqd_internal.h"
 
 namespace eqd {
 
-constexpr int kGroups = 64;
+constexpr int kGroups = 64;      // groups per tile of small grids (the minimum)
+constexpr int kMaxGroups = 1024;  // groups per tile of the largest grids
 #ifndef EQD_CANDIDATE_EVERY
 #define EQD_CANDIDATE_EVERY 4
 #endif
@@ -40,12 +42,12 @@ struct ColState {
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Block -> (tile, group) map. Workgroups are dispatched round-robin over the 8 XCDs in
-// linear block order, so XCD x receives blocks b = x, x + 8, ... With T tiles of kGroups
+// linear block order, so XCD x receives blocks b = x, x + 8, ... With T tiles of G
 // groups, the first 8*floor(T/8) tiles are dealt whole to XCDs (each XCD's L2 and its
 // share of the Infinity Cache only hold its own tiles' vectors) and the groups of the
 // remaining T % 8 tiles are spread evenly over all 8 XCDs, so every XCD carries T/8
 // tiles' work instead of some carrying one tile more. G = gridDim.x is the number of groups
-// per tile (kGroups for the vector kernels; the stencil may use fewer).
+// per tile.
 __device__ __forceinline__ void tile_group(int& tile, int& g) {
     const int G = (int)gridDim.x;  // groups per tile
     const int b = blockIdx.x + G * blockIdx.y;
@@ -164,7 +166,7 @@ __device__ __forceinline__ void block_partial(double acc, double* __restrict__ p
     if (threadIdx.x < K) {
         double s = 0.0;
         for (int i = 0; i < NPB; ++i) s += sm[i * K + threadIdx.x];
-        partial[((size_t)tile * kGroups + g) * K + threadIdx.x] = s;
+        partial[((size_t)tile * gridDim.x + g) * K + threadIdx.x] = s;
     }
 }
 
@@ -182,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_dot(OpView<T> op, int NN, cons
     if (tile_on[tile]) {
         const T* pt = p + (size_t)tile * NN * 3 * K;
         T* qt = q + (size_t)tile * NN * 3 * K;
-        for (int base = g * NPB; base < NN; base += kGroups * NPB) {
+        for (int base = g * NPB; base < NN; base += (int)gridDim.x * NPB) {
             int node = base + slot;
             if (K == 64) node = uni(node);
             if (node >= NN) continue;
@@ -218,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(OpView<double> op, int NN, 
     double acc = 0.0;
     if (tile_on[tile]) {
         const size_t off = (size_t)tile * NN * 3 * K;
-        for (int base = g * NPB; base < NN; base += kGroups * NPB) {
+        for (int base = g * NPB; base < NN; base += (int)gridDim.x * NPB) {
             int node = base + slot;
             if (K == 64) node = uni(node);
             if (node >= NN) continue;
@@ -256,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_update(int NN, int NX, int mface, X*
         const X a = (X)st.alpha;
         const T at = (T)st.alpha;
         const size_t off = (size_t)tile * NN * 3 * K;
-        for (int base = g * NPB; base < NN; base += kGroups * NPB) {
+        for (int base = g * NPB; base < NN; base += (int)gridDim.x * NPB) {
             const int node = base + slot;
             if (node >= NN) continue;
             const size_t e = off + (size_t)node * 3 * K + lane;
@@ -288,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(int NN, T* __restrict__ p, c
     if (!tile_on[tile] || st.status != kActive) return;
     const T b = (T)st.beta;
     const size_t off = (size_t)tile * NN * 3 * K;
-    for (int base = g * NPB; base < NN; base += kGroups * NPB) {
+    for (int base = g * NPB; base < NN; base += (int)gridDim.x * NPB) {
         const int node = base + slot;
         if (node >= NN) continue;
         const size_t e = off + (size_t)node * 3 * K + lane;
@@ -379,7 +381,7 @@ __device__ __forceinline__ void block_partial_vec(const double (&acc)[VW], doubl
     if (t < 64) {
         double s = 0.0;
         for (int i = 0; i < ROWS; ++i) s += sm[i][t];
-        partial[((size_t)tile * kGroups + g) * 64 + t] = s;
+        partial[((size_t)tile * gridDim.x + g) * 64 + t] = s;
     }
 }
 
@@ -411,7 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_update64(int NN, int NX, int mface, 
         const size_t nv = (size_t)NN * 192 / VW;
         V* rv = reinterpret_cast<V*>(r + off);
         const V* qv = reinterpret_cast<const V*>(q + off);
-        for (size_t v = (size_t)g * kBlock + t; v < nv; v += (size_t)kGroups * kBlock) {
+        for (size_t v = (size_t)g * kBlock + t; v < nv; v += (size_t)gridDim.x * kBlock) {
             const int node = (int)(v * VW / 192);
             const double w = (node % NX == mface) ? 2.0 : 1.0;
             V rr = sld<T>(rv + v);
@@ -459,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_pupdate64(int NN, T* __restrict__ x,
     V* xv = reinterpret_cast<V*>(x + off);
     V* pv = reinterpret_cast<V*>(p + off);
     const V* rv = reinterpret_cast<const V*>(r + off);
-    for (size_t v = (size_t)g * kBlock + t; v < nv; v += (size_t)kGroups * kBlock) {
+    for (size_t v = (size_t)g * kBlock + t; v < nv; v += (size_t)gridDim.x * kBlock) {
         V pp = sld<T>(pv + v);
         if (anyx) {
             V xx = sld<T>(xv + v);
@@ -493,7 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_xaccum(int NN, double* __restrict__ 
         return;
     }
     const size_t off = (size_t)tile * NN * 3 * K;
-    for (int base = g * NPB; base < NN; base += kGroups * NPB) {
+    for (int base = g * NPB; base < NN; base += (int)gridDim.x * NPB) {
         const int node = base + slot;
         if (node >= NN) continue;
         const size_t e = off + (size_t)node * 3 * K + lane;
@@ -505,10 +507,19 @@ __global__ __launch_bounds__(kBlock) void k_xaccum(int NN, double* __restrict__ 
     }
 }
 
+// Sum of a column's G group partials in group order (G is a multiple of 16): each chunk's
+// 16 loads are issued before its adds, so the adds wait on one round trip per chunk.
 template <int K>
-__device__ __forceinline__ double sum_partials(const double* __restrict__ partial, int tile, int lane) {
+__device__ __forceinline__ double sum_partials(const double* __restrict__ partial, int G, int tile, int lane) {
+    const double* p = partial + (size_t)tile * G * K + lane;
     double s = 0.0;
-    for (int g = 0; g < kGroups; ++g) s += partial[((size_t)tile * kGroups + g) * K + lane];
+    for (int g0 = 0; g0 < G; g0 += 16) {
+        double v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = p[(size_t)(g0 + j) * K];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) s += v[j];
+    }
     return s;
 }
 
@@ -518,7 +529,7 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ partia
 // mode 1: after the update (convergence test, beta, iteration count, itmax)
 // mode 2: after a reliable-update residual (mixed precision only)
 template <int K>
-__global__ void k_finalize(int mode, int mixed, int rel_interval, ColState* __restrict__ cs,
+__global__ void k_finalize(int G, int mode, int mixed, int rel_interval, ColState* __restrict__ cs,
                            const double* __restrict__ partial, int* __restrict__ tile_on,
                            int* __restrict__ tile_rel, int* __restrict__ active_count) {
     const int tile = blockIdx.x, lane = threadIdx.x;
@@ -526,7 +537,7 @@ __global__ void k_finalize(int mode, int mixed, int rel_interval, ColState* __re
     const double meps = 2.220446049250313e-16;
     if (mode == 0) st.xpend = 0;  // the previous iteration's p update has run
     if (st.status == kActive) {
-        const double s = sum_partials<K>(partial, tile, lane);
+        const double s = sum_partials<K>(partial, G, tile, lane);
         if (mode == 0) {
             if (s <= meps * st.pnorm2 && fabs(s) <= meps * st.pnorm2) {
                 st.status = kZeroCurv;
@@ -622,7 +633,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(RhsView rv, int NN, const double
     const double* s2 = fields + (size_t)col * 2 * n23 * n23;
     const double* s3 = s2 + (size_t)n23 * n23;
     const size_t off = (size_t)tile * NN * 3 * K;
-    for (int base = g * NPB; base < NN; base += kGroups * NPB) {
+    for (int base = g * NPB; base < NN; base += (int)gridDim.x * NPB) {
         const int node = base + slot;
         if (node >= NN) continue;
         const int m = node % rv.NX, yz = node / rv.NX, y = yz % rv.NY, z = yz / rv.NY;
@@ -658,7 +669,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(RhsView rv, int NN, const double
 
 // Initial per-column state from ||b|| (weighted): Krylov's setup before its loop.
 template <int K>
-__global__ void k_init_state(ColState* __restrict__ cs, const double* __restrict__ partial,
+__global__ void k_init_state(int G, ColState* __restrict__ cs, const double* __restrict__ partial,
                              const int64_t* __restrict__ colid, double rtol, double atol, int64_t itmax,
                              int* __restrict__ tile_on, const int* __restrict__ mask) {
     const int tile = blockIdx.x, lane = threadIdx.x;
@@ -666,7 +677,7 @@ __global__ void k_init_state(ColState* __restrict__ cs, const double* __restrict
     // Columns outside the mask (already in flight) keep their state. No early return: the
     // tile's activity below is one reduction over every lane of the block.
     if (!mask || mask[(size_t)tile * K + lane]) {
-        const double s = sum_partials<K>(partial, tile, lane);
+        const double s = sum_partials<K>(partial, G, tile, lane);
         st.col = colid[(size_t)tile * K + lane];
         st.rr = s;
         st.pnorm2 = s;
@@ -697,7 +708,7 @@ __global__ __launch_bounds__(kBlock) void k_selfdot(int NN, int NX, int mface, c
     const int lane = threadIdx.x % K, slot = threadIdx.x / K;
     const size_t off = (size_t)tile * NN * 3 * K;
     double acc = 0.0;
-    for (int base = g * NPB; base < NN; base += kGroups * NPB) {
+    for (int base = g * NPB; base < NN; base += (int)gridDim.x * NPB) {
         const int node = base + slot;
         if (node >= NN) continue;
         const size_t e = off + (size_t)node * 3 * K + lane;

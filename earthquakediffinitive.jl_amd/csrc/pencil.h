@@ -336,7 +336,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(F ? EQD_
         const size_t off = (size_t)tile * NN * 3 * 64;
         const __amdgpu_buffer_rsrc_t rs = eqd_rsrc(in + off, (uint32_t)((size_t)NN * 3 * 64 * sizeof(T)));
         const int np = op.NY * op.NY * op.nseg;
-        for (int pi = g * 4 + wv; pi < np; pi += kGroups * 4) {
+        for (int pi = g * 4 + wv; pi < np; pi += (int)gridDim.x * 4) {
             const int pidx = uni(pi);
             const int seg = pidx % op.nseg, yz = pidx / op.nseg, y = yz % op.NY, z = yz / op.NY;
             const int m0 = seg * S;
