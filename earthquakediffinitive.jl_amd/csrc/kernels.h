@@ -43,11 +43,12 @@ __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlan
 
 // Block -> (tile, group) map. Workgroups are dispatched round-robin over the 8 XCDs in
 // linear block order, so XCD x receives blocks b = x, x + 8, ... With T tiles of G
-// groups, the first 8*floor(T/8) tiles are dealt whole to XCDs (each XCD's L2 and its
-// share of the Infinity Cache only hold its own tiles' vectors) and the groups of the
-// remaining T % 8 tiles are spread evenly over all 8 XCDs, so every XCD carries T/8
-// tiles' work instead of some carrying one tile more. G = gridDim.x is the number of groups
-// per tile.
+// groups (G = gridDim.x, a multiple of 64), the first 8*floor(T/8) tiles are dealt whole
+// to XCDs (each XCD's L2 only holds its own tiles' vectors) and the groups of the remaining
+// T % 8 tiles are split evenly over all 8 XCDs, so every XCD carries T/8 tiles' work instead
+// of some carrying one tile more. Each XCD takes a contiguous run of those groups: group g
+// works on pencils (and nodes) next to those of g +- 1, so an XCD's neighbour lines stay in
+// its own L2.
 __device__ __forceinline__ void tile_group(int& tile, int& g) {
     const int G = (int)gridDim.x;  // groups per tile
     const int b = blockIdx.x + G * blockIdx.y;
@@ -57,7 +58,8 @@ __device__ __forceinline__ void tile_group(int& tile, int& g) {
         tile = ((q / G) << 3) + xcd;
         g = q % G;
     } else {
-        const int r = ((q - whole) << 3) + xcd;  // 0 .. (T % 8) * G - 1
+        const int per = (int)(gridDim.y & 7u) * (G >> 3);  // remainder blocks per XCD
+        const int r = xcd * per + (q - whole);              // 0 .. (T % 8) * G - 1
         tile = (int)(gridDim.y & ~7u) + r / G;
         g = r % G;
     }
