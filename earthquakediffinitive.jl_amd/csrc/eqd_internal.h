@@ -68,6 +68,7 @@ void sbp4_neumann_1d(int n, double h, std::vector<double>& L, std::vector<double
 template <typename T>
 struct PencilView {
     int NX, NY, mface, nseg;
+    int by;  // x2 rows per block of the pencil order (NY: whole rows)
     const T* AXs;  // NX x (2*kRxS+1)
     const T* AXo;  // NX x (2*kRxO+1)
     const T* MX;   // NX x (2*kRxO+1)

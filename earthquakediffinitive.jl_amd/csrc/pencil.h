@@ -28,6 +28,10 @@
 #ifndef EQD_NT_Q
 #define EQD_NT_Q 1
 #endif
+// x2 rows per block of the pencil order on large grids (see PencilView::by)
+#ifndef EQD_PENCIL_BY
+#define EQD_PENCIL_BY 8
+#endif
 
 namespace eqd {
 
@@ -338,7 +342,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(F ? EQD_
         const int np = op.NY * op.NY * op.nseg;
         for (int pi = g * 4 + wv; pi < np; pi += (int)gridDim.x * 4) {
             const int pidx = uni(pi);
-            const int seg = pidx % op.nseg, yz = pidx / op.nseg, y = yz % op.NY, z = yz / op.NY;
+            // pencil order: x2 blocks of op.by rows one after another; within a block the
+            // x1 segment fastest, then x2, then x3 (op.by = NY: whole x2 rows)
+            const int full = op.by * op.NY * op.nseg;
+            const int yb = pidx / full;
+            int rem = pidx - yb * full;
+            const int by = min(op.by, op.NY - yb * op.by);
+            const int seg = rem % op.nseg;
+            rem /= op.nseg;
+            const int y = yb * op.by + rem % by, z = rem / by;
             const int m0 = seg * S;
             T o[3][S];
             pencil_apply<T, S, F>(op, xb, rs, m0, y, z, lane, o);
