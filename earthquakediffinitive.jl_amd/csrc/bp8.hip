@@ -406,6 +406,54 @@ __global__ void k_max(const double* __restrict__ v, int n, double* __restrict__ 
     if (threadIdx.x == 0) *out = red[0];
 }
 
+// write_outputs's derived fields of one evaluated state (src/BP8.jl:617-630), one block:
+// go[0] = max |V|, go[1] = mu sum_i w_i |V_i| (moment rate; fixed-order block reduction), and
+// for every listed node k = idx[i]: fo[6 i ..] = V2, V3, tau2, tau3, q2, q3 with the Darcy
+// velocity q_j = kfac (D1_j p)(k), kfac = -k / viscosity (src/PorePressure.jl:52-56).
+__global__ __launch_bounds__(256) void k_outputs(Dev d, const double* __restrict__ u, const double* __restrict__ D1,
+                                                 const double* __restrict__ w, double mu, double kfac,
+                                                 const int* __restrict__ idx, int nidx, double* __restrict__ fo,
+                                                 double* __restrict__ go) {
+    __shared__ double smax[256], ssum[256];
+    const int nf = d.c.nf, n2f = d.c.n2f;
+    double mx = 0.0, sm = 0.0;
+    for (int i = threadIdx.x; i < nf; i += 256) {
+        mx = fmax(mx, d.Vm[i]);
+        sm += w[i] * d.Vm[i];
+    }
+    smax[threadIdx.x] = mx;
+    ssum[threadIdx.x] = sm;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+            ssum[threadIdx.x] += ssum[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && go) {
+        go[0] = smax[0];
+        go[1] = mu * ssum[0];
+    }
+    if (!fo) return;
+    const double* p = u + 3 * (size_t)nf;
+    for (int i = threadIdx.x; i < nidx; i += 256) {
+        const int k = idx[i], a = k % n2f, b = k / n2f;
+        double q2 = 0.0, q3 = 0.0;
+        for (int j = 0; j < n2f; ++j) {
+            q2 += D1[(size_t)a * n2f + j] * p[j + (size_t)n2f * b];
+            q3 += D1[(size_t)b * n2f + j] * p[a + (size_t)n2f * j];
+        }
+        double* o = fo + (size_t)i * 6;
+        o[0] = d.V2[k];
+        o[1] = d.V3[k];
+        o[2] = d.tau2[k];
+        o[3] = d.tau3[k];
+        o[4] = kfac * q2;
+        o[5] = kfac * q3;
+    }
+}
+
 __global__ void k_reset_cache(Dev d) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= d.c.nf) return;
@@ -1541,6 +1589,41 @@ eqd_status eqd_bp8_saved_vmax(eqd_bp8* m, double* vmax, int64_t n) {
         HIPCHK(hipGetLastError());
         check_newton(*m);
         HIPCHK(hipMemcpy(vmax, out.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+eqd_status eqd_bp8_saved_outputs(eqd_bp8* m, int64_t n, const int* idx, int nidx, double* out_fields,
+                                 double* out_global) {
+    return guard([&] {
+        if (!m) throw Error(EQD_ERR_ARG, "null model");
+        if (n > m->nsaved) throw Error(EQD_ERR_ARG, "more states requested than saved");
+        if (nidx < 0 || (nidx > 0 && !idx)) throw Error(EQD_ERR_ARG, "bad node index list");
+        for (int i = 0; i < nidx; ++i)
+            if (idx[i] < 0 || idx[i] >= m->c.nf) throw Error(EQD_ERR_ARG, "node index outside Omega_f");
+        if (n <= 0) return;
+        const int n2f = m->c.n2f;
+        std::vector<double> D1;
+        sbp4_d1_1d(n2f, m->p.dz, D1);
+        DevMem dD1(D1.size() * 8), dw((size_t)m->c.nf * 8), didx((size_t)std::max(nidx, 1) * sizeof(int));
+        DevMem fo((size_t)std::max<int64_t>(1, n * nidx) * 6 * 8), go((size_t)n * 2 * 8);
+        HIPCHK(hipMemcpyAsync(dD1.p, D1.data(), D1.size() * 8, hipMemcpyHostToDevice, m->st));
+        HIPCHK(hipMemcpyAsync(dw.p, m->w_h.data(), (size_t)m->c.nf * 8, hipMemcpyHostToDevice, m->st));
+        if (nidx) HIPCHK(hipMemcpyAsync(didx.p, idx, (size_t)nidx * sizeof(int), hipMemcpyHostToDevice, m->st));
+        const double kfac = -m->p.k / m->p.viscosity;
+        const size_t ld = m->ldS();
+        for (int64_t j = 0; j < n; ++j) {  // evaluate! in save order, as write_outputs does
+            const double* u = m->saves.as<double>() + (size_t)j * ld;
+            rhs_device(*m, u, 0.0, m->scratch.as<double>());
+            hipLaunchKernelGGL(k_outputs, dim3(1), dim3(256), 0, m->st, m->dev(), u, dD1.as<double>(), dw.as<double>(),
+                               m->p.mu, kfac, didx.as<int>(), nidx,
+                               nidx ? fo.as<double>() + (size_t)j * nidx * 6 : nullptr, go.as<double>() + 2 * j);
+        }
+        HIPCHK(hipGetLastError());
+        check_newton(*m);
+        if (out_fields && nidx)
+            HIPCHK(hipMemcpy(out_fields, fo.p, (size_t)n * nidx * 6 * 8, hipMemcpyDeviceToHost));
+        if (out_global) HIPCHK(hipMemcpy(out_global, go.p, (size_t)n * 2 * 8, hipMemcpyDeviceToHost));
+        else HIPCHK(hipStreamSynchronize(m->st));
     });
 }
 

@@ -63,6 +63,8 @@ std::vector<double> build_band(const HostTab& t, int R);
 // Pore-pressure operator factor along one Omega_f axis (D2 + Neumann SATs) and the
 // SBP-4 quadrature weights, n x n row-major.
 void sbp4_neumann_1d(int n, double h, std::vector<double>& L, std::vector<double>& H);
+// SBP-4 first derivative on n points with spacing h, n x n row-major.
+void sbp4_d1_1d(int n, double h, std::vector<double>& D1);
 
 // Device view of the band tables used by the pencil kernel.
 template <typename T>

@@ -279,4 +279,8 @@ void sbp4_neumann_1d(int n, double h, std::vector<double>& L, std::vector<double
     H = o.H;
 }
 
+// SBP-4 first derivative D1 on n points (dense n x n row-major): the operator of the Darcy
+// velocity q_j = -(k / viscosity) D1_j p on the Omega_f grid (src/PorePressure.jl:52-56).
+void sbp4_d1_1d(int n, double h, std::vector<double>& D1) { D1 = sbp4_dense(n, h).D1; }
+
 }  // namespace eqd

@@ -243,3 +243,6 @@ def effective_stress_report(m: BP8Model) -> dict:
     radius = 0.0 if len(idx) == 0 else float(max(math.hypot(m.x2[i % n2], m.x3[i // n2]) for i in idx))
     return dict(sigma_lowest=lo.value, floor_hits=hits.value, floor=m.par["sigma_min"], bound=hits.value > 0,
                 nodes=len(idx), fraction=len(idx) / m.nf, radius=radius)
+
+
+from .outputs import saved_outputs, station_locations, write_outputs  # noqa: E402  (section 4 output files)

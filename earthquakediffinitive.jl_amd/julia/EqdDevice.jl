@@ -312,4 +312,26 @@ function effective_stress_report(m::DeviceBP8Model)
               fraction=nodes / m.nf)
 end
 
+"""
+    saved_outputs(m, sol, idx) -> (; V2, V3, τ2, τ3, q2, q3, Vmax, moment_rate)
+
+The fields write_outputs (src/BP8.jl:605-642) recomputes with evaluate! at every saved time,
+evaluated on the device in one pass over the saves of the last run_bp8: slip rates, shear
+stresses and Darcy velocities at the Ω_f nodes `idx` (1-based linear, x2 fastest) as
+`length(idx) × nsaves` matrices, and max |V| and the moment rate per save. The reference's
+writers (write_time_series, write_global, write_profiles) take these in place of their
+evaluate! loop.
+"""
+function saved_outputs(m::DeviceBP8Model, sol, idx::AbstractVector{<:Integer})
+    ns = length(sol.t)
+    ci = Cint.(idx .- 1)
+    f = zeros(6, length(ci), ns)
+    g = zeros(2, ns)
+    check(ccall((:eqd_bp8_saved_outputs, libeqd), Cint,
+                (Ptr{Cvoid}, Int64, Ptr{Cint}, Cint, Ptr{Float64}, Ptr{Float64}),
+                m.handle, ns, ci, length(ci), f, g))
+    return (; V2=f[1, :, :], V3=f[2, :, :], τ2=f[3, :, :], τ3=f[4, :, :], q2=f[5, :, :], q3=f[6, :, :],
+              Vmax=g[1, :], moment_rate=g[2, :])
+end
+
 end # module

@@ -226,6 +226,14 @@ eqd_status eqd_bp8_saved_device(const eqd_bp8* m, const double** us_dev, int64_t
 /* max_i |V_i| at every saved state of the last integrate, each evaluated with evaluate!
  * in save order as write_outputs does (src/BP8.jl:628). */
 eqd_status eqd_bp8_saved_vmax(eqd_bp8* m, double* vmax, int64_t n);
+/* The derived fields write_outputs computes at every saved state (src/BP8.jl:617-630): the
+ * first n saves of the last integrate are evaluated with evaluate! in save order. For each save
+ * j, out_global[2j] = max_i |V_i| and out_global[2j+1] = mu sum_i w_i |V_i| (moment rate, w the
+ * Omega_f quadrature weights); for each listed Omega_f node k = idx[i] (0-based, x2 fastest),
+ * out_fields[(j nidx + i) 6 + f] holds V2, V3, tau2, tau3, darcy_vel_2, darcy_vel_3 with
+ * q_j = -(k / viscosity) D1_j p (src/PorePressure.jl:52-56). Either output may be NULL. */
+eqd_status eqd_bp8_saved_outputs(eqd_bp8* m, int64_t n, const int* idx, int nidx, double* out_fields,
+                                 double* out_global);
 
 /* ------------------------------------------------------------------------------
  * Trajectory ensemble: M copies of run_bp8 (src/BP8.jl:442-470) on one model's K, A_p and
