@@ -191,6 +191,15 @@ def bench_ensemble(args):
     t0 = time.perf_counter()
     m = bp8.build_model(par=par, dz=20.0, L_fault=800.0, L_normal=400.0, stiffness="toeplitz", precision="mixed")
     t_build = time.perf_counter() - t0
+    # one trajectory over the same horizon: K s from the Toeplitz separation kernel (build_model's
+    # model) and from the dense 1.38 GB K (the same K handed to model_from_stiffness)
+    single = {}
+    for name, mm in (("toeplitz_kernel", m), ("dense_K", bp8.model_from_stiffness(m.K, m.n2f, par=m.par))):
+        sol = bp8.run_bp8(mm, tspan=(0.0, args.ens_hours * 3600.0), saveat=3600.0)
+        single[name] = {"device_loop_ms": round(sol.ms, 1), "rhs_evaluations": sol.nrhs,
+                        "us_per_rhs": round(sol.ms * 1e3 / max(1, sol.nrhs), 2), "accepted_steps": sol.naccept,
+                        "final_slip_centre": float(sol.u[-1, (m.n2f // 2) * (m.n2f + 1)])}
+        del mm
     mem = perturbed_members(m.par, args.ens_members, seed=20261203)
     e = Ensemble(m, args.ens_members, **mem)
     t0 = time.perf_counter()
@@ -202,7 +211,7 @@ def bench_ensemble(args):
     res = {"nf": m.nf, "members": args.ens_members, "hours": args.ens_hours, "K_build_ms": round(t_build * 1e3, 1),
            "run_ms": round(t_run * 1e3, 1), "device_loop_ms": round(e.loop_ms, 1),
            "accepted_steps_min_max": [int(min(acc)), int(max(acc))],
-           "member_steps_per_s": round(sum(acc) / (e.loop_ms * 1e-3), 1),
+           "member_steps_per_s": round(sum(acc) / (e.loop_ms * 1e-3), 1), "single_trajectory": single,
            "gemm": {"bound": "mfma", "achieved": round(tf / 1e12, 2), "peak": 78.6, "unit": "TFLOP/s",
                     "frac": round(tf / 78.6e12, 4), "ms_per_gemm": round(ms_gemm, 3), "flop_per_gemm": flops}}
     print(f"bench: ensemble dz=20 {args.ens_members} members {args.ens_hours} h: K build {t_build * 1e3:.0f} ms, "

@@ -77,6 +77,7 @@ struct Ctl {
 struct Dev {
     Consts c;
     const double* KR;      // K row-major, 2nf x 2nf
+    const double* Tk;      // Toeplitz K: 4 blocks per separation (see k_toeplitz_kernel), or null
     const double* Lx;      // n2f x n2f row-major (alpha not applied)
     const double* src;     // nf
     const unsigned char* active;
@@ -148,15 +149,34 @@ __global__ __launch_bounds__(256) void k_bp8_stage(Dev d, int stage, const doubl
     const double t = stage != 0 ? d.ctl->t + kC[stage] * dt : t_fixed;
 
     // dtau = K s: rows i and nf + i
-    const int nh = c.n2 >> 1;
-    const double2* r2 = reinterpret_cast<const double2*>(d.KR + (size_t)i * c.n2);
-    const double2* r3 = reinterpret_cast<const double2*>(d.KR + (size_t)(c.nf + i) * c.n2);
-    const double2* sv = reinterpret_cast<const double2*>(y);
     double a2 = 0.0, a3 = 0.0;
-    for (int j = lane; j < nh; j += 64) {
-        const double2 s = sv[j], k2 = r2[j], k3 = r3[j];
-        a2 += k2.x * s.x + k2.y * s.y;
-        a3 += k3.x * s.x + k3.y * s.y;
+    if (d.Tk) {
+        // Toeplitz K (src/FaultResponse.jl:378-390): K's four nf x nf blocks depend on the node
+        // separation only, so row i is a window of the separation kernel: K s is a 2D
+        // correlation read from the kernel (4 (2 n2f - 1)^2 values, cache-resident) instead of
+        // 2 rows of K from HBM.
+        const int n2 = c.n2f, W2 = 2 * n2 - 1;
+        const int ia = i % n2, ib = i / n2;
+        const int base = (ia + n2 - 1) + W2 * (ib + n2 - 1);
+        const float inv = 1.0f / (float)n2;  // exact row index for j < 1.6e6 (checked on the host)
+        const double4* T = reinterpret_cast<const double4*>(d.Tk);
+        for (int j = lane; j < c.nf; j += 64) {
+            const int bj = (int)(((float)j + 0.5f) * inv), aj = j - bj * n2;
+            const double4 k = T[base - aj - W2 * bj];
+            const double s2 = y[j], s3 = y[c.nf + j];
+            a2 += k.x * s2 + k.z * s3;
+            a3 += k.y * s2 + k.w * s3;
+        }
+    } else {
+        const int nh = c.n2 >> 1;
+        const double2* r2 = reinterpret_cast<const double2*>(d.KR + (size_t)i * c.n2);
+        const double2* r3 = reinterpret_cast<const double2*>(d.KR + (size_t)(c.nf + i) * c.n2);
+        const double2* sv = reinterpret_cast<const double2*>(y);
+        for (int j = lane; j < nh; j += 64) {
+            const double2 s = sv[j], k2 = r2[j], k3 = r3[j];
+            a2 += k2.x * s.x + k2.y * s.y;
+            a3 += k3.x * s.x + k3.y * s.y;
+        }
     }
     const double dtau2 = wave_sum(a2), dtau3 = wave_sum(a3);
 
@@ -391,6 +411,24 @@ __global__ void k_transpose(const double* __restrict__ K, long long ldK, int n, 
         const int i = bx + r, j = by + threadIdx.x;  // KR[i n + j] = K(i, j)
         if (i < n && j < n) KR[(size_t)i * n + j] = tile[threadIdx.x][r];
     }
+}
+
+// Separation kernel of a Toeplitz K (column-major K, leading dimension ldK): for node
+// separation (da, db), key = (da + n - 1) + (2n - 1)(db + n - 1), Tk[4 key + c] holds the
+// entries K[i, j] (c = 0), K[nf + i, j] (1), K[i, nf + j] (2), K[nf + i, nf + j] (3) of any
+// pair with i - j = (da, db); the pair taken is i = (max(da, 0), max(db, 0)).
+__global__ void k_toeplitz_kernel(const double* __restrict__ K, long long ldK, int n, double* __restrict__ Tk) {
+    const int W = 2 * n - 1;
+    const int key = blockIdx.x * blockDim.x + threadIdx.x;
+    if (key >= W * W) return;
+    const int da = key % W - (n - 1), db = key / W - (n - 1);
+    const long long nf = (long long)n * n;
+    const long long i = (da > 0 ? da : 0) + (long long)n * (db > 0 ? db : 0);
+    const long long j = (da < 0 ? -da : 0) + (long long)n * (db < 0 ? -db : 0);
+    Tk[4 * (size_t)key + 0] = K[i + j * ldK];
+    Tk[4 * (size_t)key + 1] = K[nf + i + j * ldK];
+    Tk[4 * (size_t)key + 2] = K[i + (nf + j) * ldK];
+    Tk[4 * (size_t)key + 3] = K[nf + i + (nf + j) * ldK];
 }
 
 __global__ void k_max(const double* __restrict__ v, int n, double* __restrict__ out) {
@@ -840,7 +878,7 @@ struct eqd_bp8 {
     Consts c{};
     std::vector<double> Lx_h, src_h, w_h;
     std::vector<unsigned char> active_h;
-    DevMem KR, Lx, src, active, V2, V3, Vm, tau2, tau3, Vprev, sig_low, floor_hits, floor_mask, fail;
+    DevMem KR, Tk, Lx, src, active, V2, V3, Vm, tau2, tau3, Vprev, sig_low, floor_hits, floor_mask, fail;
     DevMem k, uprev, unew, ya, yb, partial, ctl, saves, save_t, scratch;
     int64_t save_cap = 0, nsaved = 0;
     hipStream_t st = nullptr;
@@ -857,6 +895,7 @@ struct eqd_bp8 {
         Dev d;
         d.c = c;
         d.KR = KR.as<double>();
+        d.Tk = Tk.p ? Tk.as<double>() : nullptr;
         d.Lx = Lx.as<double>();
         d.src = src.as<double>();
         d.active = active.as<unsigned char>();
@@ -1447,6 +1486,16 @@ eqd_status eqd_bp8_create_from_solver(eqd_solver* s, int stiffness, const eqd_bp
         const eqd_status rc = eqd_fault_stiffness_device(s, stiffness, nullptr, 0, K.as<double>(), 2 * nf);
         if (rc != EQD_OK) throw Error(rc, g_err);
         set_K_device(*m, K.as<double>(), 2 * nf);
+        if (stiffness == EQD_STIFFNESS_TOEPLITZ && nf < 1000000) {
+            // the time loop reads K s from the separation kernel (the stage kernel's row
+            // index uses a float reciprocal, exact below 1.6e6 nodes)
+            const int W = 2 * n2f - 1;
+            m->Tk.alloc((size_t)W * W * 4 * 8);
+            hipLaunchKernelGGL(k_toeplitz_kernel, dim3((W * W + 255) / 256), dim3(256), 0, m->st, K.as<double>(),
+                               (long long)(2 * nf), n2f, m->Tk.as<double>());
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(m->st));
+        }
         *out = m.release();
     });
 }
