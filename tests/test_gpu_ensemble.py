@@ -134,3 +134,32 @@ def test_bad_arguments(toeplitz_K):
     ens = Ensemble(m, 2)
     with pytest.raises(eqd.EqdError):
         ens.run(tspan=(0.0, 3600.0), saveat=600.0, member_reltol=[1e-8, 0.0])
+
+
+def test_production_scale_ensemble_matches_single_models():
+    """The dz = 20 m ensemble configuration (nf = 6561, K 13122^2 from the device Toeplitz build,
+    K S as the 13122 x 13122 x M MFMA GEMM): member 0 (base parameters) against run_bp8 of the
+    same model (K s from the separation kernel), and saved V_max of two perturbed members against
+    evaluate! of one-member dense-K models with their parameters."""
+    print("[ensemble dz=20 nf=6561 M=16 3 h]", flush=True)
+    par = bp8.benchmark_parameters(l_f=800.0, dz=20.0)
+    m = bp8.build_model(par=par, dz=20.0, L_fault=800.0, L_normal=400.0, stiffness="toeplitz", precision="mixed")
+    mp = perturbed_members(m.par, 16, seed=11)
+    ens = Ensemble(m, 16, **mp)
+    ens.run(tspan=(0.0, 3 * 3600.0), saveat=1800.0)
+    s0 = ens.solution(0)
+    single = bp8.run_bp8(m, tspan=(0.0, 3 * 3600.0), saveat=1800.0)
+    nf = m.nf
+    assert np.array_equal(s0.t, single.t)
+    assert close(s0.u[:, :2 * nf], single.u[:, :2 * nf], 1e-6, 1e-12)
+    assert close(s0.u[:, 3 * nf:4 * nf], single.u[:, 3 * nf:4 * nf], 1e-6, 1e-3)
+    n = len(s0.t)
+    vmax, _ = ens.saved_vmax(n)
+    K = m.K
+    for j in (5, 15):
+        mj = bp8.model_from_stiffness(K, m.n2f, par=bp8.benchmark_parameters(
+            l_f=800.0, dz=20.0, Q0=float(mp["Q0"][j]), a=float(mp["a"][j]), sigma0=float(mp["sigma0"][j])))
+        sol = ens.solution(j)
+        ref = np.array([bp8.evaluate(mj, u, t)["Vmag"].max() for t, u in zip(sol.t, sol.u)])
+        assert close(vmax[j], ref, 1e-10), j
+        del mj

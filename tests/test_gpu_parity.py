@@ -191,3 +191,26 @@ def test_autotuned_tile_width():
     K2 = eqd.fault_stiffness(fe)  # tuned width is cached
     assert eqd.solver_report(fe.rs)["tile_k"] == k
     assert np.array_equal(K, K2)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_empty_ragged_and_repeated_column_lists(precision):
+    """No columns -> an empty K block and no solve; a ragged list (65 columns: one full 64-lane
+    tile and one lane of the next, with a repeated index) -> the full build's columns, bitwise;
+    an index outside 0..2nf-1 -> an argument error, as the reference's bounds check."""
+    fe, o = pair("ring", precision=precision, tile_k=64)  # 2nf = 578 columns
+    K_all = eqd.fault_stiffness(fe)
+    nf2 = 2 * o.nf
+    fe2, _ = pair("ring", precision=precision, tile_k=64)
+    K0 = eqd.fault_stiffness(fe2, cols=[])
+    assert K0.shape == (nf2, 0)
+    assert eqd.solver_report(fe2.rs)["solves"] == 0
+    rng = np.random.default_rng(3)
+    cols = list(rng.permutation(nf2)[:64]) + [int(rng.integers(nf2))]
+    cols[10] = cols[20]
+    Kc = eqd.fault_stiffness(fe2, cols=cols)
+    assert np.array_equal(Kc, K_all[:, cols])
+    with pytest.raises(eqd.EqdError):
+        eqd.fault_stiffness(fe2, cols=[nf2])
+    with pytest.raises(eqd.EqdError):
+        eqd.fault_stiffness(fe2, cols=[-1])
