@@ -15,7 +15,9 @@ module EqdDevice
 export DeviceCGSolver, DeviceFaultElasticity, split_node_solve, duplicate, merge_stats!, solver_report,
        shear_traction, fault_stiffness, fault_stiffness_toeplitz, frictional_node_count, fault_grid_axes,
        DeviceBP8Model, build_model, initial_state, evaluate!, rhs!, run_bp8, state_length,
-       effective_stress_report
+       effective_stress_report, saved_outputs, DeviceEnsemble, run_ensemble!, member_solution,
+       ensemble_saved_vmax, DeviceServer, add_grid!, submit_columns!, submit_forward!, serve_step!,
+       completed_requests, request_result
 
 const libeqd = joinpath(@__DIR__, "..", "libeqd.so")
 
@@ -332,6 +334,142 @@ function saved_outputs(m::DeviceBP8Model, sol, idx::AbstractVector{<:Integer})
                 m.handle, ns, ci, length(ci), f, g))
     return (; V2=f[1, :, :], V3=f[2, :, :], τ2=f[3, :, :], τ3=f[4, :, :], q2=f[5, :, :], q3=f[6, :, :],
               Vmax=g[1, :], moment_rate=g[2, :])
+end
+
+# ---------------------------------------------------------------------------------------
+# Trajectory ensembles (no counterpart in the reference, which runs one trajectory per
+# call): M members of run_bp8 on one model's K, each with its own Q0, a, σ0 (and reltol);
+# K S over all members is one FP64 MFMA GEMM per stage.
+
+mutable struct DeviceEnsemble
+    handle::Ptr{Cvoid}
+    model::DeviceBP8Model
+    members::Int
+    stats::Vector{BP8Stats}
+end
+
+function DeviceEnsemble(m::DeviceBP8Model, members::Integer; Q0=nothing, a=nothing, σ0=nothing)
+    arr(v) = v === nothing ? C_NULL : (length(v) == members ? Float64.(collect(v)) :
+                                        error("member parameter arrays need one value per member"))
+    q, aa, sg = arr(Q0), arr(a), arr(σ0)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve q aa sg begin
+        check(ccall((:eqd_bp8_ensemble_create, libeqd), Cint,
+                    (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ref{Ptr{Cvoid}}),
+                    m.handle, members, q, aa, sg, h))
+    end
+    e = DeviceEnsemble(h[], m, members, BP8Stats[])
+    finalizer(x -> ccall((:eqd_bp8_ensemble_destroy, libeqd), Cint, (Ptr{Cvoid},), x.handle), e)
+    return e
+end
+
+"Integrates every member (run_bp8's controller per member); returns the members' stats."
+function run_ensemble!(e::DeviceEnsemble; tspan=(0.0, e.model.par.t_f), reltol=1e-8, member_reltol=nothing,
+                       saveat=3600.0, maxiters=100000)
+    t0, tf = Float64.(tspan)
+    nmax = floor(Int, (tf - t0) / saveat + 1e-9) + 2
+    st = [BP8Stats() for _ in 1:e.members]
+    rt = member_reltol === nothing ? C_NULL : Float64.(collect(member_reltol))
+    ms = Ref{Cdouble}(0)
+    GC.@preserve rt begin
+        check(ccall((:eqd_bp8_ensemble_integrate, libeqd), Cint,
+                    (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Ptr{Cdouble}, Int64, Int64, Ptr{BP8Stats},
+                     Ref{Cdouble}), e.handle, t0, tf, saveat, reltol, rt, maxiters, nmax, st, ms))
+    end
+    e.stats = st
+    return st
+end
+
+"Member j's (1-based) saves as (; t, u) like run_bp8 returns."
+function member_solution(e::DeviceEnsemble, j::Integer)
+    n = Ref{Int64}(0)
+    check(ccall((:eqd_bp8_ensemble_saved, libeqd), Cint, (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ref{Int64}),
+                e.handle, j - 1, C_NULL, C_NULL, n))
+    ts = zeros(n[])
+    us = zeros(e.model.state_len, n[])
+    check(ccall((:eqd_bp8_ensemble_saved, libeqd), Cint, (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ref{Int64}),
+                e.handle, j - 1, ts, us, n))
+    return (; t=ts, u=[us[:, k] for k in 1:n[]])
+end
+
+"max |V| at the first n saves of every member (n × members), one GEMM per save."
+function ensemble_saved_vmax(e::DeviceEnsemble, n::Integer)
+    v = zeros(n, e.members)
+    check(ccall((:eqd_bp8_ensemble_saved_vmax, libeqd), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Int64, Ptr{Cdouble}),
+                e.handle, v, n, C_NULL))
+    return v
+end
+
+# ---------------------------------------------------------------------------------------
+# Serving loop (no counterpart in the reference, whose K build is a batch job): stiffness
+# columns and forward solves on resident grids, continuously batched.
+
+mutable struct DeviceServer
+    handle::Ptr{Cvoid}
+end
+
+function DeviceServer(; rtol=1e-10, precision::Symbol=:mixed, hbm_budget_bytes::Integer=0)
+    o = CgOpts()
+    ccall((:eqd_cg_opts_default, libeqd), Cvoid, (Ref{CgOpts},), o)
+    o.rtol = rtol
+    o.precision = EQD_PREC[precision]
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:eqd_server_create, libeqd), Cint, (Ref{CgOpts}, Csize_t, Ref{Ptr{Cvoid}}),
+                o, hbm_budget_bytes, h))
+    sv = DeviceServer(h[])
+    finalizer(x -> ccall((:eqd_server_destroy, libeqd), Cint, (Ptr{Cvoid},), x.handle), sv)
+    return sv
+end
+
+"Makes a grid resident; returns (grid id, lanes)."
+function add_grid!(sv::DeviceServer, n1, n23, L_normal, L_fault, λ, μ; l_f, max_columns=0)
+    g = GridDesc(n1, n23, L_normal, L_fault, λ, μ, l_f)
+    id, slots = Ref{Cint}(0), Ref{Cint}(0)
+    check(ccall((:eqd_server_add_grid, libeqd), Cint, (Ptr{Cvoid}, Ref{GridDesc}, Cint, Ref{Cint}, Ref{Cint}),
+                sv.handle, g, max_columns, id, slots))
+    return id[], slots[]
+end
+
+"Queues the unit-slip columns `cols` (1-based) of K on a grid; returns the request id."
+function submit_columns!(sv::DeviceServer, grid::Integer, cols::AbstractVector{<:Integer})
+    c = Cint.(cols .- 1)
+    rid = Ref{Int64}(0)
+    check(ccall((:eqd_server_submit_columns, libeqd), Cint, (Ptr{Cvoid}, Cint, Ptr{Cint}, Cint, Ref{Int64}),
+                sv.handle, grid, c, length(c), rid))
+    return rid[]
+end
+
+"Queues shear_traction(fe, s2, s3) on a grid; returns the request id."
+function submit_forward!(sv::DeviceServer, grid::Integer, s2::Vector{Float64}, s3::Vector{Float64})
+    rid = Ref{Int64}(0)
+    check(ccall((:eqd_server_submit_forward, libeqd), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ref{Int64}), sv.handle, grid, s2, s3, rid))
+    return rid[]
+end
+
+"One step of every grid; returns (lanes busy, requests queued)."
+function serve_step!(sv::DeviceServer)
+    busy, queued = Ref{Int64}(0), Ref{Int64}(0)
+    check(ccall((:eqd_server_step, libeqd), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}), sv.handle, busy, queued))
+    return busy[], queued[]
+end
+
+function completed_requests(sv::DeviceServer; max=1024)
+    ids = zeros(Int64, max)
+    n = Ref{Cint}(0)
+    check(ccall((:eqd_server_completed, libeqd), Cint, (Ptr{Cvoid}, Ptr{Int64}, Cint, Ref{Cint}),
+                sv.handle, ids, max, n))
+    return ids[1:n[]]
+end
+
+"Result of a completed request: [Δτ2; Δτ3] per column (2nf × ncols), iterations, unconverged."
+function request_result(sv::DeviceServer, id::Integer, nf::Integer, ncols::Integer)
+    out = zeros(2nf, ncols)
+    iters = zeros(Cint, ncols)
+    unc = Ref{Cint}(0)
+    check(ccall((:eqd_server_result, libeqd), Cint, (Ptr{Cvoid}, Int64, Ptr{Cdouble}, Int64, Ptr{Cint}, Ref{Cint}),
+                sv.handle, id, out, 2nf, iters, unc))
+    return out, iters, unc[]
 end
 
 end # module
