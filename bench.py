@@ -310,7 +310,8 @@ def main():
     ap.add_argument("--reliable_interval", type=int, default=0,
                     help="mixed precision: iterations between FP64 residual refreshes (0 = library default)")
     ap.add_argument("--plain_cols", type=int, default=16)
-    ap.add_argument("--cpu_cols", type=int, default=64)
+    ap.add_argument("--cpu_cols", type=int, default=1024,
+                    help="columns of the CPU oracle sample (about 10 s on 16 host threads)")
     ap.add_argument("--no_cpu", action="store_true")
     ap.add_argument("--no_extras", action="store_true", help="only the timed mixed-precision steps")
     ap.add_argument("--no_bp8", action="store_true", help="skip the BP8 time-loop measurement")
