@@ -78,6 +78,8 @@ struct Dev {
     Consts c;
     const double* KR;      // K row-major, 2nf x 2nf
     const double* Tk;      // Toeplitz K: 4 blocks per separation (see k_toeplitz_kernel), or null
+    double* ks;            // with Tk: K s partials, ks_split x nf x 2 (see k_toeplitz_ks)
+    int ks_split;          // source x3 rows are split into this many chunks
     const double* Lx;      // n2f x n2f row-major (alpha not applied)
     const double* src;     // nf
     const unsigned char* active;
@@ -151,21 +153,10 @@ __global__ __launch_bounds__(256) void k_bp8_stage(Dev d, int stage, const doubl
     // dtau = K s: rows i and nf + i
     double a2 = 0.0, a3 = 0.0;
     if (d.Tk) {
-        // Toeplitz K (src/FaultResponse.jl:378-390): K's four nf x nf blocks depend on the node
-        // separation only, so row i is a window of the separation kernel: K s is a 2D
-        // correlation read from the kernel (4 (2 n2f - 1)^2 values, cache-resident) instead of
-        // 2 rows of K from HBM.
-        const int n2 = c.n2f, W2 = 2 * n2 - 1;
-        const int ia = i % n2, ib = i / n2;
-        const int base = (ia + n2 - 1) + W2 * (ib + n2 - 1);
-        const float inv = 1.0f / (float)n2;  // exact row index for j < 1.6e6 (checked on the host)
-        const double4* T = reinterpret_cast<const double4*>(d.Tk);
-        for (int j = lane; j < c.nf; j += 64) {
-            const int bj = (int)(((float)j + 0.5f) * inv), aj = j - bj * n2;
-            const double4 k = T[base - aj - W2 * bj];
-            const double s2 = y[j], s3 = y[c.nf + j];
-            a2 += k.x * s2 + k.z * s3;
-            a3 += k.y * s2 + k.w * s3;
+        // Toeplitz K: k_toeplitz_ks left one partial per source chunk; sum them in chunk order
+        for (int q = lane; q < d.ks_split; q += 64) {
+            a2 += d.ks[((size_t)q * c.nf + i) * 2];
+            a3 += d.ks[((size_t)q * c.nf + i) * 2 + 1];
         }
     } else {
         const int nh = c.n2 >> 1;
@@ -410,6 +401,51 @@ __global__ void k_transpose(const double* __restrict__ K, long long ldK, int n, 
     for (int r = threadIdx.y; r < 32; r += 8) {
         const int i = bx + r, j = by + threadIdx.x;  // KR[i n + j] = K(i, j)
         if (i < n && j < n) KR[(size_t)i * n + j] = tile[threadIdx.x][r];
+    }
+}
+
+// K s of a Toeplitz K (src/FaultResponse.jl:378-390) from its separation kernel: K's four nf x nf
+// blocks depend on the node separation only, so the contribution of source x3 row bj to target
+// x3 row ib is a 1D correlation of the kernel row (ib - bj) with the slip row bj. Block
+// (ib, chunk) stages that kernel row (2 n2f - 1 entries of 4 blocks) and the slip row in LDS for
+// every bj of its chunk and leaves its partial sums of target row ib in ks[chunk][node][2]; the
+// stage kernel adds the chunks in order. No row of K is read (at nf = 6561: 41 MB of L2 reads
+// per evaluation instead of 2.8 GB).
+__global__ __launch_bounds__(128) void k_toeplitz_ks(Dev d, int stage, const double* __restrict__ y) {
+    const Consts& c = d.c;
+    if (stage != 0 && d.ctl->done) return;
+    extern __shared__ double ks_lds[];
+    const int n2 = c.n2f, W2 = 2 * n2 - 1, nf = c.nf;
+    double4* Trow = reinterpret_cast<double4*>(ks_lds);     // W2 entries
+    double2* srow = reinterpret_cast<double2*>(Trow + W2);   // n2 entries (s2, s3)
+    const int ib = blockIdx.x, chunk = blockIdx.y;
+    const int per = (n2 + d.ks_split - 1) / d.ks_split;
+    const int b0 = chunk * per, b1 = min(n2, b0 + per);
+    const double4* T = reinterpret_cast<const double4*>(d.Tk);
+    for (int ia0 = 0; ia0 < n2; ia0 += blockDim.x) {
+        const int ia = ia0 + threadIdx.x;
+        double a2 = 0.0, a3 = 0.0;
+        for (int bj = b0; bj < b1; ++bj) {
+            __syncthreads();  // the previous row's readers are done
+            const size_t trow = (size_t)W2 * (ib - bj + n2 - 1);
+            for (int q = threadIdx.x; q < W2; q += blockDim.x) Trow[q] = T[trow + q];
+            for (int q = threadIdx.x; q < n2; q += blockDim.x)
+                srow[q] = make_double2(y[q + (size_t)n2 * bj], y[nf + q + (size_t)n2 * bj]);
+            __syncthreads();
+            if (ia < n2) {
+                for (int aj = 0; aj < n2; ++aj) {
+                    const double4 k = Trow[ia - aj + n2 - 1];
+                    const double2 sv = srow[aj];
+                    a2 += k.x * sv.x + k.z * sv.y;
+                    a3 += k.y * sv.x + k.w * sv.y;
+                }
+            }
+        }
+        if (ia < n2) {
+            const size_t o = ((size_t)chunk * nf + ia + (size_t)n2 * ib) * 2;
+            d.ks[o] = a2;
+            d.ks[o + 1] = a3;
+        }
     }
 }
 
@@ -878,7 +914,8 @@ struct eqd_bp8 {
     Consts c{};
     std::vector<double> Lx_h, src_h, w_h;
     std::vector<unsigned char> active_h;
-    DevMem KR, Tk, Lx, src, active, V2, V3, Vm, tau2, tau3, Vprev, sig_low, floor_hits, floor_mask, fail;
+    DevMem KR, Tk, ks, Lx, src, active, V2, V3, Vm, tau2, tau3, Vprev, sig_low, floor_hits, floor_mask, fail;
+    int ks_split = 0;  // source-row chunks of the Toeplitz K s (0: dense K)
     DevMem k, uprev, unew, ya, yb, partial, ctl, saves, save_t, scratch;
     int64_t save_cap = 0, nsaved = 0;
     hipStream_t st = nullptr;
@@ -896,6 +933,8 @@ struct eqd_bp8 {
         d.c = c;
         d.KR = KR.as<double>();
         d.Tk = Tk.p ? Tk.as<double>() : nullptr;
+        d.ks = ks.p ? ks.as<double>() : nullptr;
+        d.ks_split = ks_split;
         d.Lx = Lx.as<double>();
         d.src = src.as<double>();
         d.active = active.as<unsigned char>();
@@ -1077,9 +1116,18 @@ void initial_state(const eqd_bp8& m, double* u) {
     for (int i = 0; i < m.c.nf; ++i) u[2 * m.c.nf + i] = std::log(theta0);
 }
 
+// K s partials of a Toeplitz-K model for the stage about to run on y (no-op for a dense K).
+void toeplitz_ks(eqd_bp8& m, const Dev& d, int stage, const double* y) {
+    if (!m.ks_split) return;
+    const int n2 = m.c.n2f;
+    const size_t lds = (size_t)(2 * n2 - 1) * 32 + (size_t)n2 * 16;
+    hipLaunchKernelGGL(k_toeplitz_ks, dim3(n2, m.ks_split), dim3(128), lds, m.st, d, stage, y);
+}
+
 // One rhs! at a device state into a device output, then the Newton status.
 void rhs_device(eqd_bp8& m, const double* y_dev, double t, double* out_dev) {
     Dev d = m.dev();
+    toeplitz_ks(m, d, 0, y_dev);
     hipLaunchKernelGGL(k_bp8_stage, dim3(m.node_blocks()), dim3(256), 0, m.st, d, 0, y_dev, out_dev,
                        (double*)nullptr, t);
     HIPCHK(hipGetLastError());
@@ -1116,9 +1164,11 @@ void build_graph(eqd_bp8& m) {
         // stage s reads y_s and writes y_{s+1}: y2 = ya, y3 = yb, y4 = ya, y5 = yb, y6 = ya, u_new
         const double* in[8] = {nullptr, nullptr, ya, yb, ya, yb, ya, un};
         double* out[8] = {nullptr, nullptr, yb, ya, yb, ya, un, nullptr};
-        for (int stage = 2; stage <= 7; ++stage)
+        for (int stage = 2; stage <= 7; ++stage) {
+            toeplitz_ks(m, d, stage, in[stage]);
             hipLaunchKernelGGL(k_bp8_stage, dim3(m.node_blocks()), dim3(256), 0, m.st, d, stage, in[stage],
                                k + (size_t)(stage - 1) * ns, out[stage], 0.0);
+        }
         hipLaunchKernelGGL(k_bp8_control, dim3(1), dim3(256), 0, m.st, d);
         hipLaunchKernelGGL(k_bp8_commit, dim3(m.state_blocks()), dim3(256), 0, m.st, d, ya, 0);
     }
@@ -1486,10 +1536,13 @@ eqd_status eqd_bp8_create_from_solver(eqd_solver* s, int stiffness, const eqd_bp
         const eqd_status rc = eqd_fault_stiffness_device(s, stiffness, nullptr, 0, K.as<double>(), 2 * nf);
         if (rc != EQD_OK) throw Error(rc, g_err);
         set_K_device(*m, K.as<double>(), 2 * nf);
-        if (stiffness == EQD_STIFFNESS_TOEPLITZ && nf < 1000000) {
-            // the time loop reads K s from the separation kernel (the stage kernel's row
-            // index uses a float reciprocal, exact below 1.6e6 nodes)
+        if (stiffness == EQD_STIFFNESS_TOEPLITZ && n2f <= 683) {
+            // the time loop reads K s from the separation kernel: one block per (target x3 row,
+            // source chunk), about 512 blocks in all; the kernel row and the slip row take
+            // (2 n2f - 1) 32 + n2f 16 bytes of LDS (< 64 KiB up to n2f = 683)
             const int W = 2 * n2f - 1;
+            m->ks_split = std::max(1, std::min(n2f, (512 + n2f - 1) / n2f));
+            m->ks.alloc((size_t)m->ks_split * nf * 2 * 8);
             m->Tk.alloc((size_t)W * W * 4 * 8);
             hipLaunchKernelGGL(k_toeplitz_kernel, dim3((W * W + 255) / 256), dim3(256), 0, m->st, K.as<double>(),
                                (long long)(2 * nf), n2f, m->Tk.as<double>());
