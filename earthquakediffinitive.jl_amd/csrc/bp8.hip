@@ -61,6 +61,7 @@ __device__ inline void interp_weights(double th, double b[7]) {
 
 struct Consts {
     int nf, n2f, n2, nstate, injection, well_cell;
+    int lw;  // band width of the 1D pore-pressure factor rows (see band_start)
     double tau0_2, tau0_3, eta, a, b, Dc, Vs, fs, sigma0, sigma_min, alpha, D_RS;
     double src_rate;  // q0 / (beta phi): Gaussian source amplitude while injecting
     double t_off, Q0, WI, S_e, S_well;
@@ -80,7 +81,7 @@ struct Dev {
     const double* Tk;      // Toeplitz K: 4 blocks per separation (see k_toeplitz_kernel), or null
     double* ks;            // with Tk: K s partials, ks_split x nf x 2 (see k_toeplitz_ks)
     int ks_split;          // source x3 rows are split into this many chunks
-    const double* Lx;      // n2f x n2f row-major (alpha not applied)
+    const double* Lx;      // n2f x lw: band of each row of the 1D pore-pressure factor (alpha not applied)
     const double* src;     // nf
     const unsigned char* active;
     double* V2;
@@ -102,6 +103,13 @@ struct Dev {
     double* save_t;        // nsave_max
     int ldS;
 };
+
+// First column of row r's band in the n x lw banded form of the 1D pore-pressure factor (the
+// SBP-4 D2 rows and their Neumann SATs reach at most 5 columns from the diagonal).
+__host__ __device__ inline int band_start(int r, int n, int lw) {
+    const int s = r - (lw - 1) / 2;
+    return s < 0 ? 0 : (s > n - lw ? n - lw : s);
+}
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -171,11 +179,15 @@ __global__ __launch_bounds__(256) void k_bp8_stage(Dev d, int stage, const doubl
     }
     const double dtau2 = wave_sum(a2), dtau3 = wave_sum(a3);
 
-    // A_p p at node i = (a, b): alpha (sum_c Lx[a][c] p[c + n b] + sum_c Lx[b][c] p[a + n c])
+    // A_p p at node i = (a, b): alpha (sum_c Lx[a][c] p[c + n b] + sum_c Lx[b][c] p[a + n c]), the
+    // rows' nonzeros from their bands
     const int n = c.n2f, ia = i % n, ib = i / n;
     const double* P = y + 3 * c.nf;
     double ap = 0.0;
-    for (int q = lane; q < n; q += 64) ap += d.Lx[ia * n + q] * P[q + n * ib] + d.Lx[ib * n + q] * P[ia + n * q];
+    if (lane < c.lw) {
+        const int sa = band_start(ia, n, c.lw), sb = band_start(ib, n, c.lw);
+        ap = d.Lx[ia * c.lw + lane] * P[sa + lane + n * ib] + d.Lx[ib * c.lw + lane] * P[ia + n * (sb + lane)];
+    }
     ap = c.alpha * wave_sum(ap);
 
     // rate-state slip rate (every lane computes the same scalar path)
@@ -706,8 +718,10 @@ __global__ __launch_bounds__(256) void k_ens_stage(EnsDev d, int stage, const do
         }
         const int n = c.n2f, ia = i % n, ib = i / n;
         const double* P = ym + 3 * c.nf;
+        const int sa = band_start(ia, n, c.lw), sb = band_start(ib, n, c.lw);
         double ap = 0.0;
-        for (int q = 0; q < n; ++q) ap += d.Lx[ia * n + q] * P[q + n * ib] + d.Lx[ib * n + q] * P[ia + n * q];
+        for (int q = 0; q < c.lw; ++q)
+            ap += d.Lx[ia * c.lw + q] * P[sa + q + n * ib] + d.Lx[ib * c.lw + q] * P[ia + n * (sb + q)];
         ap *= c.alpha;
         const double phi = ym[2 * c.nf + i], p = P[i];
         double v2 = 0.0, v3 = 0.0, vm = 0.0;
@@ -1077,7 +1091,18 @@ void init_model(eqd_bp8& m, const eqd_bp8_params& p, int n2f, int injection) {
     HIPCHK(hipHostMalloc((void**)&m.h_ctl, sizeof(Ctl), hipHostMallocDefault));
     const size_t nf = c.nf, ns = c.nstate;
     m.KR.alloc((size_t)c.n2 * c.n2 * 8);
-    m.Lx.alloc(m.Lx_h.size() * 8);
+    // banded device copy of the 1D factor: every nonzero of row r lies in [band_start(r), +lw)
+    c.lw = std::min(11, n2f);
+    std::vector<double> Lb((size_t)n2f * c.lw);
+    for (int r = 0; r < n2f; ++r) {
+        const int s0 = band_start(r, n2f, c.lw);
+        for (int q = 0; q < n2f; ++q) {
+            const double v = m.Lx_h[(size_t)r * n2f + q];
+            if (q >= s0 && q < s0 + c.lw) Lb[(size_t)r * c.lw + (q - s0)] = v;
+            else if (v != 0.0) throw Error(EQD_ERR_DEVICE, "pore-pressure factor row exceeds its band");
+        }
+    }
+    m.Lx.alloc(Lb.size() * 8);
     m.src.alloc(nf * 8);
     m.active.alloc(nf);
     for (DevMem* v : {&m.V2, &m.V3, &m.Vm, &m.tau2, &m.tau3, &m.Vprev, &m.sig_low, &m.partial}) v->alloc(nf * 8);
@@ -1087,7 +1112,7 @@ void init_model(eqd_bp8& m, const eqd_bp8_params& p, int n2f, int injection) {
     m.k.alloc(7 * ns * 8);
     for (DevMem* v : {&m.uprev, &m.unew, &m.ya, &m.yb, &m.scratch}) v->alloc(ns * 8);
     m.ctl.alloc(sizeof(Ctl));
-    HIPCHK(hipMemcpy(m.Lx.p, m.Lx_h.data(), m.Lx_h.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m.Lx.p, Lb.data(), Lb.size() * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(m.src.p, m.src_h.data(), nf * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(m.active.p, m.active_h.data(), nf, hipMemcpyHostToDevice));
     std::memset(m.h_ctl, 0, sizeof(Ctl));
