@@ -183,7 +183,7 @@ def bench_ensemble(args):
     """Trajectory ensemble on MFMA: dz = 20 m, L_fault = 800, L_normal = 400, l_f = 800 (6561 fault
     nodes, K 13122 x 13122 FP64 = 1.38 GB from the device :toeplitz build), 256 members with seeded
     perturbations of Q0 (+-20%), a - b (+-10%) and sigma0 (+-5%), Gaussian source, integrated
-    over args.ens_hours with saveat 3600 s. K S per stage is one FP64 MFMA GEMM (2 * 13122^2 *
+    over args.ens_hours (default 30 days) with saveat 3600 s. K S per stage is one FP64 MFMA GEMM (2 * 13122^2 *
     256 = 88.2 GFLOP); its roofline is the MI355X dense FP64 matrix peak."""
     from eqd import bp8
     from eqd.ensemble import Ensemble, perturbed_members
@@ -320,7 +320,7 @@ def main():
     ap.add_argument("--prod_cols", type=int, default=512, help="exact columns of the dz = 25 m subset")
     ap.add_argument("--no_ensemble", action="store_true", help="skip the dz = 20 m trajectory ensemble")
     ap.add_argument("--ens_members", type=int, default=256)
-    ap.add_argument("--ens_hours", type=float, default=100.0)
+    ap.add_argument("--ens_hours", type=float, default=720.0, help="ensemble horizon (30 days)")
     ap.add_argument("--serve_seconds", type=float, default=10.0)
     ap.add_argument("--serve_rate", type=float, default=10.0,
                     help="mean request rate; 50 req/s over this grid mix exceeds one GPU (see DESIGN.md)")
