@@ -580,8 +580,36 @@ __global__ __launch_bounds__(256) void k_ks_gemm(int nr, int nm, int nk, const d
     constexpr int KS = BK / 4;
     __shared__ __attribute__((aligned(16))) double As[2][BK * LDA_S];
     __shared__ __attribute__((aligned(16))) double Bs[2][BN * LDK];
-    const int i0 = blockIdx.x * BM, m0 = blockIdx.y * BN;
-    const int kbeg = blockIdx.z * kchunk, kend = min(nk, kbeg + kchunk);
+    // Block -> (row block, member block, k chunk). Workgroups go round-robin over the 8 XCDs in
+    // linear order; the member blocks that share one A tile (row block, k chunk) are dealt to the
+    // same XCD one after another, so that XCD's L2 fetches the K tile once for all of them (with
+    // the grid's natural order they landed on different XCDs and K crossed the fabric once per
+    // member block). Bijective over the grid: whole groups of 8 A tiles per XCD, the remaining
+    // tiles' blocks spread evenly.
+    int rblk, mblk, kz;
+    {
+        const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
+        const int b = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+        const int tiles = nbx * (int)gridDim.z;
+        const int xcd = b & 7, q = b >> 3;
+        const int whole = (tiles >> 3) * nby;
+        int tile;
+        if (q < whole) {
+            tile = ((q / nby) << 3) + xcd;
+            mblk = q % nby;
+        } else {
+            // the per = (tiles % 8) nby remainder blocks: XCD x holds a + (x < r0) of them
+            // (per = 8 a + r0), numbered contiguously from x a + min(x, r0)
+            const int per = (tiles & 7) * nby, a = per >> 3, r0 = per & 7;
+            const int r = xcd * a + min(xcd, r0) + (q - whole);
+            tile = (tiles & ~7) + r / nby;
+            mblk = r % nby;
+        }
+        rblk = tile % nbx;
+        kz = tile / nbx;
+    }
+    const int i0 = rblk * BM, m0 = mblk * BN;
+    const int kbeg = kz * kchunk, kend = min(nk, kbeg + kchunk);
     const int t = threadIdx.x, wv = t >> 6, lane = t & 63, g = lane >> 4, r16 = lane & 15;
     const int wr = (wv >> 1) * (BM / 2), wc = (wv & 1) * (BN / 2);
     dvec4 acc[TM][TN];
@@ -655,7 +683,7 @@ __global__ __launch_bounds__(256) void k_ks_gemm(int nr, int nm, int nk, const d
         __syncthreads();
         buf ^= 1;
     }
-    double* Pz = P + (size_t)blockIdx.z * ldP * nm;
+    double* Pz = P + (size_t)kz * ldP * nm;
 #pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
