@@ -160,11 +160,26 @@ __global__ __launch_bounds__(256) void k_bp8_stage(Dev d, int stage, const doubl
 
     // dtau = K s: rows i and nf + i
     double a2 = 0.0, a3 = 0.0;
-    if (d.Tk) {
+    if (d.Tk && d.ks_split) {
         // Toeplitz K: k_toeplitz_ks left one partial per source chunk; sum them in chunk order
         for (int q = lane; q < d.ks_split; q += 64) {
             a2 += d.ks[((size_t)q * c.nf + i) * 2];
             a3 += d.ks[((size_t)q * c.nf + i) * 2 + 1];
+        }
+    } else if (d.Tk) {
+        // Toeplitz K on a small patch: row i of K is a window of the separation kernel (key =
+        // base - aj - W2 bj), read here directly (the kernel is L2-resident; a separate launch
+        // would cost more than the product)
+        const int n2 = c.n2f, W2 = 2 * n2 - 1;
+        const int base = (i % n2 + n2 - 1) + W2 * (i / n2 + n2 - 1);
+        const float inv = 1.0f / (float)n2;  // exact source row for j < nf <= 48^2
+        const double4* T = reinterpret_cast<const double4*>(d.Tk);
+        for (int j = lane; j < c.nf; j += 64) {
+            const int bj = (int)(((float)j + 0.5f) * inv), aj = j - bj * n2;
+            const double4 k = T[base - aj - W2 * bj];
+            const double s2 = y[j], s3 = y[c.nf + j];
+            a2 += k.x * s2 + k.z * s3;
+            a3 += k.y * s2 + k.w * s3;
         }
     } else {
         const int nh = c.n2 >> 1;
@@ -1593,9 +1608,10 @@ eqd_status eqd_bp8_create_from_solver(eqd_solver* s, int stiffness, const eqd_bp
             // the time loop reads K s from the separation kernel: one block per (target x3 row,
             // source chunk), about 512 blocks in all; the kernel row and the slip row take
             // (2 n2f - 1) 32 + n2f 16 bytes of LDS (< 64 KiB up to n2f = 683)
+            // (patches of up to 48 x 48 nodes read the kernel inside the stage kernel instead)
             const int W = 2 * n2f - 1;
-            m->ks_split = std::max(1, std::min(n2f, (512 + n2f - 1) / n2f));
-            m->ks.alloc((size_t)m->ks_split * nf * 2 * 8);
+            m->ks_split = n2f <= 48 ? 0 : std::max(1, std::min(n2f, (512 + n2f - 1) / n2f));
+            if (m->ks_split) m->ks.alloc((size_t)m->ks_split * nf * 2 * 8);
             m->Tk.alloc((size_t)W * W * 4 * 8);
             hipLaunchKernelGGL(k_toeplitz_kernel, dim3((W * W + 255) / 256), dim3(256), 0, m->st, K.as<double>(),
                                (long long)(2 * nf), n2f, m->Tk.as<double>());
