@@ -43,3 +43,12 @@ def test_model_from_stiffness_checks_before_device():
     with pytest.raises(eqd.EqdError) as e:
         bp8.model_from_stiffness(np.zeros((10, 10)), 17)
     assert e.value.code == _lib.EQD_ERR_ARG
+
+
+def test_derived_parameters_match_table1():
+    """test/bp8_test.jl:17-25: nu = 0.25 makes lambda = mu; c_s = sqrt(mu / rho) reproduces the
+    tabulated 3.464 km/s; alpha = k / (phi beta eta) reproduces 0.05 m^2/s."""
+    p = bp8.benchmark_parameters()
+    assert bp8.lame_lambda(p) == pytest.approx(p["mu"], rel=1e-12)
+    assert np.sqrt(p["mu"] / p["rho"]) == pytest.approx(p["c_s"], rel=1e-3)
+    assert p["k"] / (p["phi"] * p["beta"] * p["viscosity"]) == pytest.approx(p["alpha"], rel=1e-12)
