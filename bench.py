@@ -127,6 +127,47 @@ def bench_bp8(args):
     return res
 
 
+def bench_plain_config(args):
+    """The CPU-runnable configuration end to end: BP8-QD-GS at dz = 100 m with l_f = L_fault = 800 m
+    (289 fault nodes, 578 columns). As written (L_normal = 400) its n1 = 5 < 9 is rejected by
+    build_model (src/BP8.jl:199-202), so the legal L_normal = 800 is used (15,606 DOF). The exact K
+    comes from the FP64 batched CG on the device, then 30 days of Tsit5 (saveat 3600 s) on the
+    device; the CPU oracle builds the same K (assembled CSR, one column per thread) and integrates
+    it beside."""
+    from eqd import bp8
+    par = bp8.benchmark_parameters(l_f=800.0, dz=100.0)
+    t0 = time.perf_counter()
+    m = bp8.build_model(par=par, dz=100.0, L_fault=800.0, L_normal=800.0, stiffness="exact", precision="fp64")
+    t_build = time.perf_counter() - t0
+    tf, saveat = 30 * 86400.0, 3600.0
+    t0 = time.perf_counter()
+    sol = bp8.run_bp8(m, tspan=(0.0, tf), saveat=saveat)
+    t_run = time.perf_counter() - t0
+    vmax = bp8.saved_vmax(m, len(sol.t))
+    K = m.K
+    res = {"dof": m.grid_info["elastic_dofs"], "columns": 2 * m.nf, "K_build_ms": round(t_build * 1e3, 1),
+           "run_ms": round(t_run * 1e3, 1), "device_loop_ms": round(sol.ms, 1), "accepted_steps": sol.naccept,
+           "peak_V": float(vmax.max())}
+    if not args.no_cpu:
+        from oracle.oracle import OracleBP8, OracleSystem, bp8_geometry
+        ncpu = min(os.cpu_count() or 1, 16)
+        o = OracleSystem(bp8_geometry(100.0, 800.0, 800.0, 800.0))
+        t0 = time.perf_counter()
+        Ko = o.fault_stiffness(nthreads=ncpu)
+        res["cpu_K_build_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        res["cpu_threads"] = ncpu
+        res["K_rel_err_vs_oracle"] = float(np.linalg.norm(K - Ko) / np.linalg.norm(Ko))
+        ob = OracleBP8(Ko, m.n2f, dict(dz=100.0, l_f=800.0))
+        t0 = time.perf_counter()
+        ro = ob.run(0.0, tf, saveat)
+        res["cpu_run_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        res["cpu_accepted_steps"] = ro["naccept"]
+        res["vmax_max_rel_diff_vs_oracle"] = float(np.max(np.abs(vmax - ro["vmax"]) / ro["vmax"]))
+    print(f"bench: dz=100 plain config: K {t_build * 1e3:.0f} ms, 30 d {t_run * 1e3:.0f} ms, "
+          f"V_max vs oracle {res.get('vmax_max_rel_diff_vs_oracle', float('nan')):.2e}", file=sys.stderr, flush=True)
+    return res
+
+
 def bench_production(args):
     """Production-size stiffness build: dz = 25 m, L_fault = 1600, L_normal = 800, l_f = 1600
     (3,294,918 DOF, 16641 fault nodes, 33282 columns). The :toeplitz build (10 solves, K filled
@@ -452,6 +493,8 @@ def main():
     Kbuf.free()
     if not args.no_extras and not args.no_bp8:
         out["bp8_30day_gs"] = bench_bp8(args)
+    if not args.no_extras and not args.no_bp8:
+        out["plain_config_dz100"] = bench_plain_config(args)
     if not args.no_extras and not args.no_production:
         out["production_dz25"] = bench_production(args)
     if not args.no_extras and not args.no_ensemble:
