@@ -441,7 +441,7 @@ __global__ void k_transpose(const double* __restrict__ K, long long ldK, int n, 
 __global__ __launch_bounds__(128) void k_toeplitz_ks(Dev d, int stage, const double* __restrict__ y) {
     const Consts& c = d.c;
     if (stage != 0 && d.ctl->done) return;
-    extern __shared__ double ks_lds[];
+    extern __shared__ __attribute__((aligned(32))) double ks_lds[];
     const int n2 = c.n2f, W2 = 2 * n2 - 1, nf = c.nf;
     double4* Trow = reinterpret_cast<double4*>(ks_lds);     // W2 entries
     double2* srow = reinterpret_cast<double2*>(Trow + W2);   // n2 entries (s2, s3)
