@@ -235,7 +235,8 @@ def bench_ensemble(args):
     # one trajectory over the same horizon: K s from the Toeplitz separation kernel (build_model's
     # model) and from the dense 1.38 GB K (the same K handed to model_from_stiffness)
     single = {}
-    for name, mm in (("toeplitz_kernel", m), ("dense_K", bp8.model_from_stiffness(m.K, m.n2f, par=m.par))):
+    Kh = m.K
+    for name, mm in (("toeplitz_kernel", m), ("dense_K", bp8.model_from_stiffness(Kh, m.n2f, par=m.par))):
         sol = bp8.run_bp8(mm, tspan=(0.0, args.ens_hours * 3600.0), saveat=3600.0)
         single[name] = {"device_loop_ms": round(sol.ms, 1), "rhs_evaluations": sol.nrhs,
                         "us_per_rhs": round(sol.ms * 1e3 / max(1, sol.nrhs), 2), "accepted_steps": sol.naccept,
@@ -246,6 +247,21 @@ def bench_ensemble(args):
     t0 = time.perf_counter()
     st = e.run(tspan=(0.0, args.ens_hours * 3600.0), saveat=3600.0)
     t_run = time.perf_counter() - t0
+    cpu_check = None
+    if not getattr(args, "no_cpu", True):
+        # member-wise against the CPU oracle at full size: members 0 (base) and 1 over the first
+        # 2 h (the oracle's dense K product costs ~0.15 s per rhs! at nf = 6561)
+        from oracle.oracle import OracleBP8
+        ens_v, _ = e.saved_vmax(3)
+        worst = 0.0
+        t0 = time.perf_counter()
+        for j in (0, 1):
+            par_j = dict(dz=20.0, l_f=800.0, Q0=float(mem["Q0"][j]), a=float(mem["a"][j]),
+                         sigma0=float(mem["sigma0"][j]))
+            ro = OracleBP8(Kh, m.n2f, par_j).run(0.0, 7200.0, 3600.0)
+            worst = max(worst, float(np.max(np.abs(ens_v[j] - ro["vmax"]) / ro["vmax"])))
+        cpu_check = {"members": [0, 1], "hours": 2, "vmax_max_rel_diff_vs_oracle": worst,
+                     "cpu_ms": round((time.perf_counter() - t0) * 1e3, 1)}
     ms_gemm, flops = e.gemm_timing(20)
     tf = flops / (ms_gemm * 1e-3)
     acc = [s.naccept for s in st]
@@ -253,6 +269,7 @@ def bench_ensemble(args):
            "run_ms": round(t_run * 1e3, 1), "device_loop_ms": round(e.loop_ms, 1),
            "accepted_steps_min_max": [int(min(acc)), int(max(acc))],
            "member_steps_per_s": round(sum(acc) / (e.loop_ms * 1e-3), 1), "single_trajectory": single,
+           "cpu_member_check": cpu_check,
            "gemm": {"bound": "mfma", "achieved": round(tf / 1e12, 2), "peak": 78.6, "unit": "TFLOP/s",
                     "frac": round(tf / 78.6e12, 4), "ms_per_gemm": round(ms_gemm, 3), "flop_per_gemm": flops}}
     print(f"bench: ensemble dz=20 {args.ens_members} members {args.ens_hours} h: K build {t_build * 1e3:.0f} ms, "
