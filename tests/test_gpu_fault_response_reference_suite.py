@@ -97,3 +97,39 @@ def test_toeplitz_build_approximates_the_exact_one():
     assert abs(K_toep[0, 0] - K_exact[0, 0]) <= 0.05 * abs(K_exact[0, 0])
     assert eqd.solver_report(fe_t.rs)["solves"] == 10
     assert eqd.solver_report(fe_e.rs)["solves"] == 2 * nf
+
+
+def test_threaded_build_with_duplicated_solvers_matches_serial():
+    """The reference's threaded build (src/FaultResponse.jl:265-281): one duplicated solver per task,
+    the operator shared read-only, the columns split in a strided partition; here two host threads
+    drive two duplicated solvers (each with its own stream) at once. Bitwise equal to the serial
+    build, with the same total iteration count."""
+    import threading
+    fe = cube11(precision="mixed", tile_k=16)
+    K_ser = eqd.fault_stiffness(fe)
+    it_ser = eqd.solver_report(fe.rs)["iterations"]
+    fe2 = cube11(precision="mixed", tile_k=16)
+    nf2 = K_ser.shape[0]
+    parts = [list(range(0, nf2, 2)), list(range(1, nf2, 2))]
+    solvers = [fe2.rs, eqd.duplicate(fe2.rs)]
+    out = [None, None]
+    errs = []
+
+    def work(i):
+        try:
+            out[i] = eqd.fault_stiffness(fe2, cols=parts[i], solver=solvers[i])
+        except Exception as e:  # surfaced below
+            errs.append(e)
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errs, errs
+    K_thr = np.zeros_like(K_ser)
+    for i in range(2):
+        K_thr[:, parts[i]] = out[i]
+    assert np.array_equal(K_thr, K_ser)
+    merged = eqd.merge_stats(solvers[0], solvers[1])
+    assert eqd.solver_report(merged)["iterations"] == it_ser
