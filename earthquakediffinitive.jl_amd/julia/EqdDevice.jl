@@ -219,6 +219,17 @@ mutable struct BP8Stats
 end
 BP8Stats() = BP8Stats(0, 0, 0, 0, 0, 0.0)
 
+# The same record as an isbits value, for arrays passed to the C ABI (a Vector of the mutable
+# struct would hold references, not the contiguous records the library writes).
+struct BP8StatsC
+    naccept::Int64
+    nreject::Int64
+    nrhs::Int64
+    nsaved::Int64
+    retcode::Cint
+    ms::Cdouble
+end
+
 mutable struct DeviceBP8Model
     handle::Ptr{Cvoid}
     par::BP8ParamsC
@@ -345,7 +356,7 @@ mutable struct DeviceEnsemble
     handle::Ptr{Cvoid}
     model::DeviceBP8Model
     members::Int
-    stats::Vector{BP8Stats}
+    stats::Vector{BP8StatsC}
 end
 
 function DeviceEnsemble(m::DeviceBP8Model, members::Integer; Q0=nothing, a=nothing, σ0=nothing)
@@ -358,7 +369,7 @@ function DeviceEnsemble(m::DeviceBP8Model, members::Integer; Q0=nothing, a=nothi
                     (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ref{Ptr{Cvoid}}),
                     m.handle, members, q, aa, sg, h))
     end
-    e = DeviceEnsemble(h[], m, members, BP8Stats[])
+    e = DeviceEnsemble(h[], m, members, BP8StatsC[])
     finalizer(x -> ccall((:eqd_bp8_ensemble_destroy, libeqd), Cint, (Ptr{Cvoid},), x.handle), e)
     return e
 end
@@ -368,12 +379,12 @@ function run_ensemble!(e::DeviceEnsemble; tspan=(0.0, e.model.par.t_f), reltol=1
                        saveat=3600.0, maxiters=100000)
     t0, tf = Float64.(tspan)
     nmax = floor(Int, (tf - t0) / saveat + 1e-9) + 2
-    st = [BP8Stats() for _ in 1:e.members]
+    st = Vector{BP8StatsC}(undef, e.members)
     rt = member_reltol === nothing ? C_NULL : Float64.(collect(member_reltol))
     ms = Ref{Cdouble}(0)
     GC.@preserve rt begin
         check(ccall((:eqd_bp8_ensemble_integrate, libeqd), Cint,
-                    (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Ptr{Cdouble}, Int64, Int64, Ptr{BP8Stats},
+                    (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Ptr{Cdouble}, Int64, Int64, Ptr{BP8StatsC},
                      Ref{Cdouble}), e.handle, t0, tf, saveat, reltol, rt, maxiters, nmax, st, ms))
     end
     e.stats = st
