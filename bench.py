@@ -373,6 +373,7 @@ def main():
     ap.add_argument("--no_cpu", action="store_true")
     ap.add_argument("--no_extras", action="store_true", help="only the timed mixed-precision steps")
     ap.add_argument("--no_bp8", action="store_true", help="skip the BP8 time-loop measurement")
+    ap.add_argument("--no_plain", action="store_true", help="skip the dz = 100 m end-to-end configuration")
     ap.add_argument("--no_serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no_production", action="store_true", help="skip the dz = 25 m production-size build")
     ap.add_argument("--prod_cols", type=int, default=512, help="exact columns of the dz = 25 m subset")
@@ -510,7 +511,7 @@ def main():
     Kbuf.free()
     if not args.no_extras and not args.no_bp8:
         out["bp8_30day_gs"] = bench_bp8(args)
-    if not args.no_extras and not args.no_bp8:
+    if not args.no_extras and not args.no_bp8 and not args.no_plain:
         out["plain_config_dz100"] = bench_plain_config(args)
     if not args.no_extras and not args.no_production:
         out["production_dz25"] = bench_production(args)
