@@ -9,6 +9,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -105,6 +106,33 @@ struct eqd_op {
     PencilView<float> p32{};
     int G = kGroups;  // workgroups per tile of the node-parallel kernels (set at creation)
     int64_t lin(int i1, int i2, int i3) const { return i1 + (int64_t)g.n1 * (i2 + (int64_t)g.n23 * i3); }
+
+    // Jacobi M^{-1} on the merged unknowns (node-major, component fastest), on the host and
+    // in HBM in both precisions; built by the first solver that asks for :jacobi.
+    std::mutex jac_mu;
+    std::vector<double> dinv_h;
+    const double* dinv64 = nullptr;
+    const float* dinv32 = nullptr;
+    void ensure_jacobi() {
+        std::lock_guard<std::mutex> lk(jac_mu);
+        if (dinv64) return;
+        std::vector<double> d = merged_diagonal(tb);
+        for (double& v : d) {
+            // every stored unknown is off the far field, so its diagonal is positive; a
+            // non-positive entry means A is not SPD (src/ElasticitySplitNode.jl:352-357)
+            if (!(v > 0)) throw Error(EQD_ERR_ARG, "diag(A) has non-positive entries; A is not PSD");
+            v = 1.0 / v;
+        }
+        std::vector<float> f(d.begin(), d.end());
+        const float* d32 = up(f);
+        dinv64 = up(d);
+        dinv32 = d32;
+        dinv_h.swap(d);
+    }
+    template <typename V>
+    const V* dinv() const {
+        if constexpr (std::is_same<V, float>::value) return dinv32; else return dinv64;
+    }
 };
 
 namespace {
@@ -315,6 +343,9 @@ struct Engine {
         double* part = w.partial.as<double>();
         OpView<T> v;
         if constexpr (std::is_same<T, float>::value) v = op.v32; else v = op.v64;
+        const bool jac = s->o.precond == EQD_PRECOND_JACOBI;
+        const T* dinv = jac ? op.dinv<T>() : nullptr;
+        const double* dinv64 = jac ? op.dinv64 : nullptr;
         auto timed = [&](int id, auto&& launch) {
             if (kt) kt->begin(id, st);
             launch();
@@ -326,10 +357,12 @@ struct Engine {
                 if constexpr (std::is_same<T, float>::value) pv = op.p32; else pv = op.p64;
                 if (pv.nseg == 1)
                     hipLaunchKernelGGL((k_pencil<T, T, 0, true>), grid, dim3(kBlock), pencil_lds_bytes(pv), st, pv,
-                                       op.NN, p, q, (const T*)nullptr, (T*)nullptr, cs, on, part);
+                                       op.NN, p, q, (const T*)nullptr, (T*)nullptr, cs, on, part,
+                                       (const double*)nullptr);
                 else
                     hipLaunchKernelGGL((k_pencil<T, T, 0, false>), grid, dim3(kBlock), pencil_lds_bytes(pv), st, pv,
-                                       op.NN, p, q, (const T*)nullptr, (T*)nullptr, cs, on, part);
+                                       op.NN, p, q, (const T*)nullptr, (T*)nullptr, cs, on, part,
+                                       (const double*)nullptr);
             } else {
                 hipLaunchKernelGGL((k_apply_dot<T, K>), grid, dim3(kBlock), 0, st, v, op.NN, p, q, on, part);
             }
@@ -342,12 +375,17 @@ struct Engine {
             T* xu = nullptr;
             if constexpr (std::is_same<T, float>::value) xu = w.xc.as<float>();
             else xu = w.x.as<double>();
-            if constexpr (K == 64)
-                hipLaunchKernelGGL((k_update64<T>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface, r, q, cs,
-                                   on, part);
-            else
+            if constexpr (K == 64) {
+                if (dinv)
+                    hipLaunchKernelGGL((k_update64<T, true>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface,
+                                       r, q, cs, on, part, dinv);
+                else
+                    hipLaunchKernelGGL((k_update64<T, false>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX,
+                                       op.tb.mface, r, q, cs, on, part, dinv);
+            } else {
                 hipLaunchKernelGGL((k_update<T, T, K>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface, xu,
-                                   r, p, q, cs, on, part);
+                                   r, p, q, cs, on, part, dinv);
+            }
         });
         timed(3, [&] {
             hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, op.G, 1, (int)mixed, R, cs, part, on, rel,
@@ -357,10 +395,14 @@ struct Engine {
             T* xu = nullptr;
             if constexpr (std::is_same<T, float>::value) xu = w.xc.as<float>();
             else xu = w.x.as<double>();
-            if constexpr (K == 64)
-                hipLaunchKernelGGL((k_pupdate64<T>), grid, dim3(kBlock), 0, st, op.NN, xu, p, r, cs);
-            else
-                hipLaunchKernelGGL((k_pupdate<T, K>), grid, dim3(kBlock), 0, st, op.NN, p, r, cs, on);
+            if constexpr (K == 64) {
+                if (dinv)
+                    hipLaunchKernelGGL((k_pupdate64<T, true>), grid, dim3(kBlock), 0, st, op.NN, xu, p, r, cs, dinv);
+                else
+                    hipLaunchKernelGGL((k_pupdate64<T, false>), grid, dim3(kBlock), 0, st, op.NN, xu, p, r, cs, dinv);
+            } else {
+                hipLaunchKernelGGL((k_pupdate<T, K>), grid, dim3(kBlock), 0, st, op.NN, p, r, cs, on, dinv);
+            }
         });
         if (mixed) {
             timed(4, [&] {
@@ -375,15 +417,15 @@ struct Engine {
                     if (op.p64.nseg == 1)
                         hipLaunchKernelGGL((k_pencil<double, T, 1, true>), grid, dim3(kBlock),
                                            pencil_lds_bytes(op.p64), st, op.p64, op.NN, x64, (double*)nullptr, b64, r,
-                                           cs, rel, part);
+                                           cs, rel, part, dinv64);
                     else
                         hipLaunchKernelGGL((k_pencil<double, T, 1, false>), grid, dim3(kBlock),
                                            pencil_lds_bytes(op.p64), st, op.p64, op.NN, x64, (double*)nullptr, b64, r,
-                                           cs, rel, part);
+                                           cs, rel, part, dinv64);
                 }
                 else
                     hipLaunchKernelGGL((k_residual<T, K>), grid, dim3(kBlock), 0, st, op.v64, op.NN, w.x.as<double>(),
-                                       w.b.as<double>(), r, cs, rel, part);
+                                       w.b.as<double>(), r, cs, rel, part, dinv64);
             });
             timed(3, [&] {
                 hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, op.G, 2, 1, R, cs, part, on, rel,
@@ -399,7 +441,8 @@ struct Engine {
         const eqd_op& op = *s->op;
         dim3 grid(s->op->G, w.ntiles);
         hipLaunchKernelGGL((k_selfdot<double, K>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface,
-                           w.b.as<double>(), w.partial.as<double>());
+                           w.b.as<double>(), w.partial.as<double>(),
+                           s->o.precond == EQD_PRECOND_JACOBI ? op.dinv64 : nullptr);
         const int64_t itmax = s->o.itmax > 0 ? s->o.itmax : 2 * op.ntot;
         hipLaunchKernelGGL((k_init_state<K>), dim3(w.ntiles), dim3(K), 0, st, op.G, w.cs.as<ColState>(),
                            w.partial.as<double>(), w.colid.as<int64_t>(), s->o.rtol, s->o.atol, itmax,
@@ -534,6 +577,7 @@ struct Engine {
         hipStream_t st = s->stream;
         dim3 grid(op.G, 1);
         std::vector<double> part(op.G);
+        const double* dinv = s->o.precond == EQD_PRECOND_JACOBI ? op.dinv64 : nullptr;
         ColState h;
         // the state was initialised on the solver's (non-blocking) stream
         HIPCHK(hipMemcpyAsync(&h, w.cs.p, sizeof(ColState), hipMemcpyDeviceToHost, st));
@@ -566,7 +610,7 @@ struct Engine {
             HIPCHK(hipMemcpyAsync(w.cs.p, &h, sizeof(ColState), hipMemcpyHostToDevice, st));
             hipLaunchKernelGGL((k_update<double, double, 1>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface,
                                w.x.as<double>(), w.r.as<double>(), w.p.as<double>(), w.q.as<double>(),
-                               w.cs.as<ColState>(), w.tile_on.as<int>(), w.partial.as<double>());
+                               w.cs.as<ColState>(), w.tile_on.as<int>(), w.partial.as<double>(), dinv);
             const double rr = sum();
             const double rnorm = std::sqrt(rr);
             h.iters += 1;
@@ -583,7 +627,7 @@ struct Engine {
             }
             HIPCHK(hipMemcpyAsync(w.cs.p, &h, sizeof(ColState), hipMemcpyHostToDevice, st));
             hipLaunchKernelGGL((k_pupdate<double, 1>), grid, dim3(kBlock), 0, st, op.NN, w.p.as<double>(),
-                               w.r.as<double>(), w.cs.as<ColState>(), w.tile_on.as<int>());
+                               w.r.as<double>(), w.cs.as<ColState>(), w.tile_on.as<int>(), dinv);
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipMemcpyAsync(w.cs.p, &h, sizeof(ColState), hipMemcpyHostToDevice, st));
@@ -605,7 +649,8 @@ struct Engine {
         dim3 grid(s->op->G, w.ntiles);
         hipLaunchKernelGGL((k_rhs<T, K>), grid, dim3(kBlock), 0, s->stream, op.rv, op.NN, w.fields.as<double>(),
                            w.ntiles * K, w.b.as<double>(), w.x.as<double>(), w.r.as<T>(), w.p.as<T>(),
-                           std::is_same<T, float>::value ? w.xc.as<float>() : nullptr, mask);
+                           std::is_same<T, float>::value ? w.xc.as<float>() : nullptr, mask,
+                           s->o.precond == EQD_PRECOND_JACOBI ? op.dinv64 : nullptr);
         HIPCHK(hipGetLastError());
     }
 
@@ -1192,6 +1237,33 @@ eqd_status eqd_apply_A(eqd_op* op, const double* x, double* y) {
     });
 }
 
+eqd_status eqd_jacobi_preconditioner(const eqd_grid_desc* g, double* minv, int64_t n) {
+    return guard([&] {
+        if (!g || !minv) throw Error(EQD_ERR_ARG, "null argument");
+        validate_grid(*g);
+        const int n1 = g->n1, n23 = g->n23;
+        const int64_t N = (int64_t)n1 * n23 * n23;
+        if (n != 6 * N) throw Error(EQD_ERR_ARG, "minv must hold the 6 n1 n23^2 full-space entries");
+        const HostTables tb = build_tables(n1, n23, g->L_normal, g->L_fault, g->lambda, g->mu);
+        const std::vector<double> d = merged_diagonal(tb);
+        auto lin = [&](int i1, int i2, int i3) { return i1 + (int64_t)n1 * (i2 + (int64_t)n23 * i3); };
+        // far-field rows of A are zero: their diagonal entries are floored to 1
+        std::fill(minv, minv + n, 1.0);
+        for (int z = 0; z < tb.NY; ++z)
+            for (int y = 0; y < tb.NY; ++y)
+                for (int m = 0; m < tb.NX; ++m) {
+                    const int M = m + 1;  // plane on the two-block x1 line; n1 - 1 is the fault face
+                    const size_t node = (size_t)m + (size_t)tb.NX * (y + (size_t)tb.NY * z);
+                    for (int c = 0; c < 3; ++c) {
+                        const double v = d[node * 3 + c];
+                        if (!(v > 0)) throw Error(EQD_ERR_ARG, "diag(A) has non-positive entries; A is not PSD");
+                        if (M <= n1 - 1) minv[c * N + lin(M, y + 1, z + 1)] = 1.0 / v;
+                        if (M >= n1 - 1) minv[3 * N + c * N + lin(M - (n1 - 1), y + 1, z + 1)] = 1.0 / v;
+                    }
+                }
+    });
+}
+
 eqd_status eqd_solver_create(eqd_op* op, const eqd_cg_opts* opts, eqd_solver** out) {
     return guard([&] {
         if (!op || !out) throw Error(EQD_ERR_ARG, "null argument");
@@ -1199,12 +1271,12 @@ eqd_status eqd_solver_create(eqd_op* op, const eqd_cg_opts* opts, eqd_solver** o
         eqd_cg_opts o;
         eqd_cg_opts_default(&o);
         if (opts) o = *opts;
-        if (o.precond == EQD_PRECOND_JACOBI)
-            throw Error(EQD_ERR_ARG, "precond :jacobi is not provided on the device (it measured 0.92x in the reference)");
-        if (o.precond != EQD_PRECOND_NONE) throw Error(EQD_ERR_ARG, "precond must be :none or :jacobi");
+        if (o.precond != EQD_PRECOND_NONE && o.precond != EQD_PRECOND_JACOBI)
+            throw Error(EQD_ERR_ARG, "precond must be :none or :jacobi");
         if (o.precision != EQD_PREC_FP64 && o.precision != EQD_PREC_MIXED) throw Error(EQD_ERR_ARG, "bad precision");
         if (o.engine != EQD_ENGINE_BATCHED && o.engine != EQD_ENGINE_PLAIN) throw Error(EQD_ERR_ARG, "bad engine");
         if (!(o.rtol >= 0) || !(o.atol >= 0)) throw Error(EQD_ERR_ARG, "rtol and atol must be >= 0");
+        if (o.precond == EQD_PRECOND_JACOBI) op->ensure_jacobi();
         std::unique_ptr<eqd_solver> s(new eqd_solver());
         s->op = op;
         s->o = o;
@@ -1284,7 +1356,7 @@ eqd_status eqd_build_rhs(eqd_op* op, const double* s2, const double* s3, double*
         HIPCHK(hipMemcpy(df.p, f.data(), f.size() * 8, hipMemcpyHostToDevice));
         hipLaunchKernelGGL((k_rhs<double, 1>), dim3(kGroups, 1), dim3(kBlock), 0, 0, op->rv, op->NN, df.as<double>(), 1,
                            db.as<double>(), dx.as<double>(), dr.as<double>(), dp.as<double>(), (float*)nullptr,
-                           (const int*)nullptr);
+                           (const int*)nullptr, (const double*)nullptr);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(b.data(), db.p, n * 8, hipMemcpyDeviceToHost));
         embed_full(*op, b.data(), rhs);
@@ -1305,16 +1377,19 @@ eqd_status eqd_split_node_solve(eqd_solver* s, const double* rhs, double* u) {
         int64_t cid = 0;
         HIPCHK(hipMemcpy(w.colid.p, &cid, 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(w.b.p, br.data(), n * 8, hipMemcpyHostToDevice));
-        // x = 0, r = p = b
+        // x = 0, r = b, p = z = b (M^{-1} b with the Jacobi preconditioner)
+        std::vector<double> zr(br);
+        if (s->o.precond == EQD_PRECOND_JACOBI)
+            for (size_t i = 0; i < n; ++i) zr[i] = op.dinv_h[i] * br[i];
         HIPCHK(hipMemset(w.x.p, 0, n * 8));
         if (prec == EQD_PREC_MIXED) {
-            std::vector<float> bf(br.begin(), br.end());
+            std::vector<float> bf(br.begin(), br.end()), zf(zr.begin(), zr.end());
             HIPCHK(hipMemcpy(w.r.p, bf.data(), n * 4, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(w.p.p, bf.data(), n * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(w.p.p, zf.data(), n * 4, hipMemcpyHostToDevice));
             HIPCHK(hipMemset(w.xc.p, 0, n * 4));
         } else {
             HIPCHK(hipMemcpy(w.r.p, br.data(), n * 8, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(w.p.p, br.data(), n * 8, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(w.p.p, zr.data(), n * 8, hipMemcpyHostToDevice));
         }
         with_k(1, [&](auto d) {
             decltype(d)::init(s);

@@ -51,6 +51,10 @@ struct HostTables {
 
 HostTables build_tables(int n1, int n23, double L_normal, double L_fault, double lambda, double mu);
 
+// diag(A) at the rows of the merged unknowns (node-major, component fastest), the Jacobi
+// preconditioner's M = diag(A) on range(P).
+std::vector<double> merged_diagonal(const HostTables& t);
+
 // Fixed-reach band form of a table: band[r*(2R+1) + R + d] = coefficient of column r + d.
 // The reaches below hold for every grid (they are set by the SBP-4 closures restricted
 // to interior columns); build_band() throws if a table exceeds its reach.

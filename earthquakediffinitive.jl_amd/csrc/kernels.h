@@ -208,12 +208,13 @@ __global__ __launch_bounds__(kBlock) void k_apply_dot(OpView<T> op, int NN, cons
 }
 
 // Reliable-update residual in FP64: r = b - A x, stored to r (type T) for flagged
-// columns, with the weighted partial r.r taken on the FP64 values.
+// columns, with the weighted partial r.r (r.M^{-1}r with a Jacobi dinv) taken on the
+// FP64 values.
 template <typename T, int K>
 __global__ __launch_bounds__(kBlock) void k_residual(OpView<double> op, int NN, const double* __restrict__ x,
                                                      const double* __restrict__ b, T* __restrict__ r,
                                                      const ColState* __restrict__ cs, const int* __restrict__ tile_on,
-                                                     double* __restrict__ partial) {
+                                                     double* __restrict__ partial, const double* __restrict__ dinv) {
     constexpr int NPB = kBlock / K;
     int tile, g;
     tile_group(tile, g);
@@ -236,7 +237,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(OpView<double> op, int NN, 
             for (int c = 0; c < 3; ++c) {
                 double rv = b[e + c * K] - o[c];
                 if (mine) r[e + c * K] = (T)rv;
-                d += rv * rv;
+                if (dinv) d += rv * (dinv[(size_t)node * 3 + c] * rv);
+                else d += rv * rv;
             }
             acc += w * d;
         }
@@ -244,12 +246,14 @@ __global__ __launch_bounds__(kBlock) void k_residual(OpView<double> op, int NN, 
     block_partial<K>(acc, partial, tile, g);
 }
 
-// x += alpha p; r -= alpha q; partial r.r   (X = solution accumulator type)
+// x += alpha p; r -= alpha q; partial r.r, or r.z with z = dinv r (Jacobi)
+// (X = solution accumulator type)
 template <typename T, typename X, int K>
 __global__ __launch_bounds__(kBlock) void k_update(int NN, int NX, int mface, X* __restrict__ x,
                                                    T* __restrict__ r, const T* __restrict__ p,
                                                    const T* __restrict__ q, const ColState* __restrict__ cs,
-                                                   const int* __restrict__ tile_on, double* __restrict__ partial) {
+                                                   const int* __restrict__ tile_on, double* __restrict__ partial,
+                                                   const T* __restrict__ dinv) {
     constexpr int NPB = kBlock / K;
     int tile, g;
     tile_group(tile, g);
@@ -272,7 +276,12 @@ __global__ __launch_bounds__(kBlock) void k_update(int NN, int NX, int mface, X*
                 x[i] += a * (X)p[i];
                 T rv = r[i] - at * q[i];
                 r[i] = rv;
-                d += (double)rv * (double)rv;
+                if (dinv) {
+                    const T zv = dinv[(size_t)node * 3 + c] * rv;
+                    d += (double)rv * (double)zv;
+                } else {
+                    d += (double)rv * (double)rv;
+                }
             }
             acc += w * d;
         }
@@ -280,10 +289,11 @@ __global__ __launch_bounds__(kBlock) void k_update(int NN, int NX, int mface, X*
     block_partial<K>(acc, partial, tile, g);
 }
 
-// p = r + beta p
+// p = z + beta p, z = r (no preconditioner) or dinv r (Jacobi)
 template <typename T, int K>
 __global__ __launch_bounds__(kBlock) void k_pupdate(int NN, T* __restrict__ p, const T* __restrict__ r,
-                                                    const ColState* __restrict__ cs, const int* __restrict__ tile_on) {
+                                                    const ColState* __restrict__ cs, const int* __restrict__ tile_on,
+                                                    const T* __restrict__ dinv) {
     constexpr int NPB = kBlock / K;
     int tile, g;
     tile_group(tile, g);
@@ -296,8 +306,13 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(int NN, T* __restrict__ p, c
         const int node = base + slot;
         if (node >= NN) continue;
         const size_t e = off + (size_t)node * 3 * K + lane;
+        if (dinv) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) p[e + c * K] = r[e + c * K] + b * p[e + c * K];
+            for (int c = 0; c < 3; ++c) p[e + c * K] = dinv[(size_t)node * 3 + c] * r[e + c * K] + b * p[e + c * K];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) p[e + c * K] = r[e + c * K] + b * p[e + c * K];
+        }
     }
 }
 
@@ -390,10 +405,13 @@ __device__ __forceinline__ void block_partial_vec(const double (&acc)[VW], doubl
 // r -= alpha q and the weighted partial r.r, for 64-column tiles with 16-byte accesses.
 // (x += alpha p is applied by k_pupdate64, which reads p anyway: 8 vector passes per
 // iteration instead of 9.)
-template <typename T>
+// PRE: Jacobi, the partial is r.z with z = dinv r (dinv per node and component, shared by
+// the 64 columns of a row).
+template <typename T, bool PRE>
 __global__ __launch_bounds__(kBlock) void k_update64(int NN, int NX, int mface, T* __restrict__ r,
                                                      const T* __restrict__ q, const ColState* __restrict__ cs,
-                                                     const int* __restrict__ tile_on, double* __restrict__ partial) {
+                                                     const int* __restrict__ tile_on, double* __restrict__ partial,
+                                                     const T* __restrict__ dinv) {
     using V = typename Vec16<T>::type;
     constexpr int VW = Vec16<T>::n, TPR = 64 / VW;
     int tile, g;
@@ -420,12 +438,15 @@ __global__ __launch_bounds__(kBlock) void k_update64(int NN, int NX, int mface, 
             const double w = (node % NX == mface) ? 2.0 : 1.0;
             V rr = sld<T>(rv + v);
             const V qq = sld<T>(qv + v);
+            T dz = T(1);
+            if constexpr (PRE) dz = dinv[v * VW / 64];
 #pragma unroll
             for (int j = 0; j < VW; ++j) {
                 if (act[j]) {
                     const T rn = vget<T>(rr, j) - a[j] * vget<T>(qq, j);
                     vset<T>(rr, j, rn);
-                    acc[j] += w * ((double)rn * (double)rn);
+                    if constexpr (PRE) acc[j] += w * ((double)rn * (double)(dz * rn));
+                    else acc[j] += w * ((double)rn * (double)rn);
                 }
             }
             sst<T>(rv + v, rr);
@@ -436,9 +457,11 @@ __global__ __launch_bounds__(kBlock) void k_update64(int NN, int NX, int mface, 
 
 // x += alpha p (columns updated this iteration) and p = r + beta p (columns still
 // active), for 64-column tiles with 16-byte accesses.
-template <typename T>
+// PRE: Jacobi, p = dinv r + beta p.
+template <typename T, bool PRE>
 __global__ __launch_bounds__(kBlock) void k_pupdate64(int NN, T* __restrict__ x, T* __restrict__ p,
-                                                      const T* __restrict__ r, const ColState* __restrict__ cs) {
+                                                      const T* __restrict__ r, const ColState* __restrict__ cs,
+                                                      const T* __restrict__ dinv) {
     using V = typename Vec16<T>::type;
     constexpr int VW = Vec16<T>::n, TPR = 64 / VW;
     int tile, g;
@@ -474,9 +497,16 @@ __global__ __launch_bounds__(kBlock) void k_pupdate64(int NN, T* __restrict__ x,
         }
         if (anyp) {
             const V rr = sld<T>(rv + v);
+            if constexpr (PRE) {
+                const T dz = dinv[v * VW / 64];
 #pragma unroll
-            for (int j = 0; j < VW; ++j)
-                if (pu[j]) vset<T>(pp, j, vget<T>(rr, j) + b[j] * vget<T>(pp, j));
+                for (int j = 0; j < VW; ++j)
+                    if (pu[j]) vset<T>(pp, j, dz * vget<T>(rr, j) + b[j] * vget<T>(pp, j));
+            } else {
+#pragma unroll
+                for (int j = 0; j < VW; ++j)
+                    if (pu[j]) vset<T>(pp, j, vget<T>(rr, j) + b[j] * vget<T>(pp, j));
+            }
             sst<T>(pv + v, pp);
         }
     }
@@ -620,11 +650,13 @@ __device__ __forceinline__ double tab_at(const Tab<double>& t, int row, int col)
 // chi = -+s/2 across the fault (src/FaultResponse.jl:176-185); the x1 structure of
 // every term reduces to the jump coefficients Jt (tangential components, from s
 // itself) and Jn (normal component, from the strike/dip derivatives of s).
-// Also initialises the Krylov vectors: x = 0, xc = 0, r = p = b.
+// Also initialises the Krylov vectors: x = 0, xc = 0, r = b, p = z = b (dinv b with a
+// Jacobi dinv).
 template <typename T, int K>
 __global__ __launch_bounds__(kBlock) void k_rhs(RhsView rv, int NN, const double* __restrict__ fields, int nslots,
                                                 double* __restrict__ b, double* __restrict__ x, T* __restrict__ r,
-                                                T* __restrict__ p, float* __restrict__ xc, const int* __restrict__ mask) {
+                                                T* __restrict__ p, float* __restrict__ xc, const int* __restrict__ mask,
+                                                const double* __restrict__ dinv) {
     constexpr int NPB = kBlock / K;
     int tile, g;
     tile_group(tile, g);
@@ -663,7 +695,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(RhsView rv, int NN, const double
             b[e + c * K] = v[c];
             x[e + c * K] = 0.0;
             r[e + c * K] = (T)v[c];
-            p[e + c * K] = (T)v[c];
+            p[e + c * K] = dinv ? (T)(dinv[(size_t)node * 3 + c] * v[c]) : (T)v[c];
             if (xc) xc[e + c * K] = 0.0f;
         }
     }
@@ -700,10 +732,11 @@ __global__ void k_init_state(int G, ColState* __restrict__ cs, const double* __r
     if (lane == 0) tile_on[tile] = on;
 }
 
-// Weighted partial dot of one vector with itself (used for ||b||).
+// Weighted partial dot of one vector with itself (used for ||b||), or v.(dinv v) with a
+// Jacobi dinv (Krylov's initial sqrt(b.M^{-1}b)).
 template <typename T, int K>
 __global__ __launch_bounds__(kBlock) void k_selfdot(int NN, int NX, int mface, const T* __restrict__ v,
-                                                    double* __restrict__ partial) {
+                                                    double* __restrict__ partial, const double* __restrict__ dinv) {
     constexpr int NPB = kBlock / K;
     int tile, g;
     tile_group(tile, g);
@@ -716,7 +749,11 @@ __global__ __launch_bounds__(kBlock) void k_selfdot(int NN, int NX, int mface, c
         const size_t e = off + (size_t)node * 3 * K + lane;
         double d = 0.0;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) d += (double)v[e + c * K] * (double)v[e + c * K];
+        for (int c = 0; c < 3; ++c) {
+            const double vv = (double)v[e + c * K];
+            if (dinv) d += vv * (dinv[(size_t)node * 3 + c] * vv);
+            else d += vv * vv;
+        }
         acc += ((node % NX == mface) ? 2.0 : 1.0) * d;
     }
     block_partial<K>(acc, partial, tile, g);

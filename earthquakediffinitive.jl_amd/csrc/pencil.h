@@ -313,12 +313,13 @@ __device__ __forceinline__ void pencil_apply(const PencilView<T>& op, const XBan
 
 // MODE 0: q = A p with the weighted partial p.q.
 // MODE 1: reliable-update residual r = b - A x (FP64 in, r stored as R for flagged
-//         lanes) with the weighted partial r.r of the FP64 residual.
+//         lanes) with the weighted partial r.r of the FP64 residual (r.(dinv r) when a
+//         Jacobi dinv is given).
 template <typename T, typename R, int MODE, bool F>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(F ? EQD_PENCIL_WAVES : EQD_PENCIL_WAVES - 1))) void k_pencil(
     PencilView<T> op, int NN, const T* __restrict__ in, T* __restrict__ q, const T* __restrict__ b,
     R* __restrict__ rout, const ColState* __restrict__ cs, const int* __restrict__ tile_on,
-    double* __restrict__ partial) {
+    double* __restrict__ partial, const double* __restrict__ dinv) {
     constexpr int S = pencil_s<T>();
     int tile, g;
     tile_group(tile, g);
@@ -376,7 +377,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(F ? EQD_
                         } else {
                             const double rv = (double)b[e + c * 64] - (double)v;
                             if (mine) rout[e + c * 64] = (R)rv;
-                            d += rv * rv;
+                            if (dinv) d += rv * (dinv[(size_t)(m + op.NX * (y + op.NY * z)) * 3 + c] * rv);
+                            else d += rv * rv;
                         }
                     }
                     acc += (m == op.mface ? 2.0 : 1.0) * d;

@@ -266,6 +266,36 @@ HostTables build_tables(int n1, int n23, double L_normal, double L_fault, double
     return T;
 }
 
+namespace {
+double tab_entry(const HostTab& t, int row, int col) {
+    const int j = col - t.start[row];
+    return (j >= 0 && j < t.width[row]) ? t.c[(size_t)row * kTabW + j] : 0.0;
+}
+}  // namespace
+
+// diag(A), A = -HP(D+SAT)P, at the rows the merged unknowns stand for, node-major with the
+// component fastest. Only the same-axis, same-component tables reach the diagonal:
+// component 0 meets AXs along x1 and To along x2 and x3, component c = 1, 2 meets AXo along
+// x1 and Ts along its own axis; the mixed terms couple different components. A face unknown
+// carries the fault pair (rm, rp): rows rm and rp of A are equal, so A[rm,rp] = A[rm,rm] and
+// the merged operator's diagonal there is 2 A[rm,rm] (src/ElasticitySplitNode.jl:306-322).
+std::vector<double> merged_diagonal(const HostTables& t) {
+    const int NX = t.NX, NY = t.NY;
+    std::vector<double> d((size_t)NX * NY * NY * 3);
+    for (int z = 0; z < NY; ++z)
+        for (int y = 0; y < NY; ++y)
+            for (int m = 0; m < NX; ++m) {
+                const double hw = -(t.Hx[m] * t.Hy[y] * t.Hy[z]) * (m == t.mface ? 0.5 : 1.0);
+                const double tyo = tab_entry(t.To, y, y), tzo = tab_entry(t.To, z, z);
+                const double a[3] = {tab_entry(t.AXs, m, m) + tyo + tzo,
+                                     tab_entry(t.AXo, m, m) + tab_entry(t.Ts, y, y) + tzo,
+                                     tab_entry(t.AXo, m, m) + tyo + tab_entry(t.Ts, z, z)};
+                const size_t node = (size_t)m + (size_t)NX * (y + (size_t)NY * z);
+                for (int c = 0; c < 3; ++c) d[node * 3 + c] = hw * a[c];
+            }
+    return d;
+}
+
 // One-dimensional factor of the pore-pressure operator on the Omega_f grid
 // (src/PorePressure.jl:36-43): the narrow D2 plus the Neumann (no-flux) SATs,
 // L = D2 + e_0 d_l / H_0 - e_N d_r / H_N, dense n x n row-major; H gets the weights.

@@ -25,7 +25,7 @@ from ._lib import EqdError, check, lib
 __all__ = [
     "GridPair", "bp8_grid_pair", "FaultElasticity", "CGSolver", "split_node_solve", "duplicate",
     "merge_stats", "solver_report", "shear_traction", "fault_stiffness", "fault_stiffness_toeplitz",
-    "frictional_node_count", "fault_grid_axes", "elastic_solver_report", "EqdError",
+    "frictional_node_count", "fault_grid_axes", "elastic_solver_report", "jacobi_preconditioner", "EqdError",
 ]
 
 
@@ -184,6 +184,18 @@ class FaultElasticity:
         check(lib().eqd_build_rhs(self.op.handle, np.ascontiguousarray(s2, np.float64),
                                   np.ascontiguousarray(s3, np.float64), rhs))
         return rhs
+
+
+def jacobi_preconditioner(fe) -> np.ndarray:
+    """jacobi_preconditioner(A) (src/ElasticitySplitNode.jl:343-357): the diagonal of
+    M^{-1} = diag(A)^{-1} over the full space, A's zero diagonal entries (the far-field rows)
+    floored to 1. `fe` is a FaultElasticity, a CGSolver or an operator; computed on the host
+    from the operator tables."""
+    op = fe.op if hasattr(fe, "op") else fe
+    n = int(op.ntot)
+    minv = np.zeros(n)
+    check(lib().eqd_jacobi_preconditioner(C.byref(op.desc), minv, n))
+    return minv
 
 
 def frictional_node_count(fe: FaultElasticity) -> int:

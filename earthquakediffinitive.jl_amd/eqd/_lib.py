@@ -23,7 +23,8 @@ STIFFNESS_EXACT, STIFFNESS_TOEPLITZ = 0, 1
 # Every symbol include/eqd.h declares (checked by tests/test_capi_symbols.py).
 EXPORTED = [
     "eqd_cg_opts_default", "eqd_operator_create", "eqd_operator_destroy", "eqd_operator_info",
-    "eqd_fault_grid_axes", "eqd_apply_A", "eqd_build_rhs", "eqd_solver_create", "eqd_solver_destroy",
+    "eqd_fault_grid_axes", "eqd_apply_A", "eqd_build_rhs", "eqd_jacobi_preconditioner", "eqd_solver_create",
+    "eqd_solver_destroy",
     "eqd_solver_duplicate", "eqd_solver_merge_stats", "eqd_solver_report", "eqd_split_node_solve",
     "eqd_shear_traction", "eqd_fault_stiffness", "eqd_fault_stiffness_device", "eqd_solver_last_timing", "eqd_solver_kernel_timing",
     "eqd_tables_export", "eqd_version", "eqd_last_error",
@@ -102,6 +103,7 @@ def lib():
         L.eqd_operator_info.argtypes = [vp, C.POINTER(OpInfo)]
         L.eqd_fault_grid_axes.argtypes = [vp, dp, dp]
         L.eqd_apply_A.argtypes = [vp, dp, dp]
+        L.eqd_jacobi_preconditioner.argtypes = [C.POINTER(GridDesc), dp, C.c_int64]
         L.eqd_build_rhs.argtypes = [vp, dp, dp, dp]
         L.eqd_solver_create.argtypes = [vp, C.POINTER(CgOpts), C.POINTER(vp)]
         L.eqd_solver_destroy.argtypes = [vp]

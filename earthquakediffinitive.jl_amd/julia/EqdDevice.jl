@@ -17,7 +17,7 @@ export DeviceCGSolver, DeviceFaultElasticity, split_node_solve, duplicate, merge
        DeviceBP8Model, build_model, initial_state, evaluate!, rhs!, run_bp8, state_length,
        effective_stress_report, saved_outputs, DeviceEnsemble, run_ensemble!, member_solution,
        ensemble_saved_vmax, DeviceServer, add_grid!, submit_columns!, submit_forward!, serve_step!,
-       completed_requests, request_result
+       completed_requests, request_result, jacobi_preconditioner
 
 const libeqd = joinpath(@__DIR__, "..", "libeqd.so")
 
@@ -97,6 +97,7 @@ mutable struct DeviceFaultElasticity
     x2f::Vector{Float64}
     x3f::Vector{Float64}
     Ntot::Int
+    grid::GridDesc
 end
 
 function _solver(op::Ptr{Cvoid}; rtol=1e-10, atol=0.0, itmax=0, precond::Symbol=:none,
@@ -128,12 +129,25 @@ function DeviceFaultElasticity(n1, n23, L_normal, L_fault, λ, μ; l_f, solver_k
     x2 = zeros(info[].n2f)
     x3 = zeros(info[].n3f)
     check(ccall((:eqd_fault_grid_axes, libeqd), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), op[], x2, x3))
-    fe = DeviceFaultElasticity(op[], _solver(op[]; solver_kwargs...), info[], x2, x3, Int(info[].ntot))
+    fe = DeviceFaultElasticity(op[], _solver(op[]; solver_kwargs...), info[], x2, x3, Int(info[].ntot), g)
     finalizer(x -> ccall((:eqd_operator_destroy, libeqd), Cint, (Ptr{Cvoid},), x.op), fe)
     return fe
 end
 
 frictional_node_count(fe::DeviceFaultElasticity) = Int(fe.info.nf)
+
+"""
+    jacobi_preconditioner(fe) -> Vector{Float64}
+
+The diagonal of jacobi_preconditioner(A)'s M⁻¹ = diag(A)⁻¹ (src/ElasticitySplitNode.jl:343-357)
+for fe's split-node A, with A's zero diagonal entries (the far-field rows) floored to 1.
+"""
+function jacobi_preconditioner(fe::DeviceFaultElasticity)
+    minv = zeros(fe.Ntot)
+    check(ccall((:eqd_jacobi_preconditioner, libeqd), Cint, (Ref{GridDesc}, Ptr{Float64}, Int64),
+                fe.grid, minv, fe.Ntot))
+    return minv
+end
 fault_grid_axes(fe::DeviceFaultElasticity) = (fe.x2f, fe.x3f)
 
 function split_node_solve(s::DeviceCGSolver, rhs::Vector{Float64})

@@ -21,6 +21,10 @@ using LinearAlgebra, Test
     @test st[nf+1:2nf] ≈ Δτ3 rtol = 1e-8
     @test solver_report(fe.rs).unconverged == 0
     @test_throws ErrorException DeviceFaultElasticity(13, 13, 1.2, 1.2, 2.0, 1.0; l_f=0.55)
+    fj = DeviceFaultElasticity(13, 13, 1.2, 1.2, 2.0, 1.0; l_f=0.6, precond=:jacobi)
+    @test fault_stiffness(fj) ≈ K rtol = 1e-8
+    minv = jacobi_preconditioner(fj)
+    @test length(minv) == fj.Ntot && all(>(0), minv)
 
     m = build_model(; Δz=100.0, L_fault=800.0, L_normal=800.0)
     u0 = initial_state(m)

@@ -64,7 +64,7 @@ typedef struct {
     double rtol;              /* default 1e-10 (src/ElasticitySplitNode.jl:365) */
     double atol;              /* default 0 */
     int64_t itmax;            /* 0 = 2*ntot, Krylov's default */
-    int precond;              /* EQD_PRECOND_NONE only; JACOBI is rejected (measured 0.92x in the reference) */
+    int precond;              /* EQD_PRECOND_NONE or EQD_PRECOND_JACOBI (M^{-1} = diag(A)^{-1}, src/ElasticitySplitNode.jl:343-366) */
     int precision;            /* EQD_PREC_FP64 or EQD_PREC_MIXED (FP32 CG inside FP64 reliable updates) */
     int engine;               /* EQD_ENGINE_BATCHED (device-side alpha/beta) or _PLAIN (host alpha/beta, one column) */
     int tile_k;               /* columns interleaved per tile: 1, 8, 16, 32 or 64; 0 = autotuned: the first
@@ -98,7 +98,14 @@ eqd_status eqd_apply_A(eqd_op* op, const double* x, double* y);
 /* rhs = HP(D+SAT) chi(s) for slip on Omega_f (src/FaultResponse.jl:202-203), full space. */
 eqd_status eqd_build_rhs(eqd_op* op, const double* s2, const double* s3, double* rhs);
 
-/* CGSolver(A; rtol, atol, itmax, precond) (src/ElasticitySplitNode.jl:365). */
+/* jacobi_preconditioner(A) (src/ElasticitySplitNode.jl:343-357): minv[n] = diag(A)^{-1} of
+ * A = -HP(D+SAT)P over the full space (n = 6 n1 n23^2), A's zero diagonal entries (the
+ * far-field rows) floored to 1. Host only (from the operator tables; no device needed). */
+eqd_status eqd_jacobi_preconditioner(const eqd_grid_desc* g, double* minv, int64_t n);
+
+/* CGSolver(A; rtol, atol, itmax, precond) (src/ElasticitySplitNode.jl:365). precond JACOBI
+ * runs preconditioned CG with Krylov's cg! semantics under M: gamma = r.M^{-1}r, stop at
+ * sqrt(gamma) <= atol + rtol sqrt(b.M^{-1}b). */
 eqd_status eqd_solver_create(eqd_op* op, const eqd_cg_opts* opts, eqd_solver** out);
 eqd_status eqd_solver_destroy(eqd_solver* s);
 /* duplicate(solver): shares the operator, fresh workspace and counters (:408). */
