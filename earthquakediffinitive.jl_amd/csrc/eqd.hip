@@ -15,6 +15,11 @@
 #include <vector>
 
 #include "../../include/eqd.h"
+
+// FP32 pencils per workgroup that set the groups per tile of a grid (see eqd_operator_create)
+#ifndef EQD_PENCILS_PER_GROUP
+#define EQD_PENCILS_PER_GROUP 4
+#endif
 #include "eqd_internal.h"
 #include "kernels.h"
 
@@ -1176,13 +1181,18 @@ eqd_status eqd_operator_create(const eqd_grid_desc* g, eqd_op** out) {
         op->v32 = op->view<float>();
         op->p64 = op->pview<double>();
         op->p32 = op->pview<float>();
-        // Groups per tile: about 16 FP32 pencils per workgroup (4 per wave), between 64 and
-        // 1024, so that one tile of a large grid fills the device (dz = 25 m: 64,516 pencils ->
-        // 1024 groups) while small grids keep 64. Depends on the grid only, never on the batch.
+        // Groups per tile: about 4 FP32 pencils per workgroup (one per wave), between 64 and
+        // 1024. With one pencil per wave the blocks of a tile sweep its pencils in order, so the
+        // pencils in flight on an XCD are one window of neighbours (dz = 50 m: 961 pencils ->
+        // 256 groups; stencil 640 -> 558 us per launch against 64 groups of 4 pencils per wave,
+        // whose strided sweeps keep two tiles' windows in each L2; 512 groups leave half the
+        // waves idle, 602 us). One tile of a large grid fills the device (dz = 25 m: 64,516
+        // pencils -> the 1024 cap, which bounds the per-column partial sums of k_finalize).
+        // Depends on the grid only, never on the batch.
         {
             const int np = op->p32.NY * op->p32.NY * op->p32.nseg;
             int G = kGroups;
-            while (G < kMaxGroups && G * 16 < np) G *= 2;
+            while (G < kMaxGroups && G * EQD_PENCILS_PER_GROUP < np) G *= 2;
             op->G = G;
         }
         op->rv = RhsView{op->tb.NX, op->tb.NY, g->n23, lo, cnt, op->nf, op->up(op->tb.Jt), op->up(op->tb.Jn),

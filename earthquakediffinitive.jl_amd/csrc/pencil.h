@@ -323,6 +323,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(F ? EQD_
     constexpr int S = pencil_s<T>();
     int tile, g;
     tile_group(tile, g);
+    // A block of an inactive tile leaves before staging anything: k_finalize reads the
+    // partials of active columns only, and every column of an inactive tile is inactive.
+    if (!tile_on[tile]) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     extern __shared__ unsigned char eqd_lds[];
     T* lds = reinterpret_cast<T*>(eqd_lds);
@@ -336,7 +339,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(F ? EQD_
     __syncthreads();
     const XBands<T> xb{lds, lds + nS, lds + nS + nO};
     double acc = 0.0;
-    if (tile_on[tile]) {
+    {
         const bool mine = MODE == 0 || cs[(size_t)tile * 64 + lane].rel_flag != 0;
         const size_t off = (size_t)tile * NN * 3 * 64;
         const __amdgpu_buffer_rsrc_t rs = eqd_rsrc(in + off, (uint32_t)((size_t)NN * 3 * 64 * sizeof(T)));
