@@ -110,6 +110,8 @@ struct eqd_op {
     PencilView<double> p64{};
     PencilView<float> p32{};
     int G = kGroups;  // workgroups per tile of the node-parallel kernels (set at creation)
+    int nfold = 0;    // > 0: block totals of the two-level partial sum (G > kFoldAbove)
+    int fchunk = 0;   // groups per first-level block
     int64_t lin(int i1, int i2, int i3) const { return i1 + (int64_t)g.n1 * (i2 + (int64_t)g.n23 * i3); }
 
     // Jacobi M^{-1} on the merged unknowns (node-major, component fastest), on the host and
@@ -218,7 +220,7 @@ void slip_fields(const eqd_op& op, const double* s2, const double* s3, double* f
 struct Workspace {
     int K = 0, ntiles = 0, prec = -1;
     size_t nvec = 0;
-    DevMem b, x, r, p, q, xc, fields, partial, cs, tile_on, tile_rel, colid, outpos;
+    DevMem b, x, r, p, q, xc, fields, partial, folded, cs, tile_on, tile_rel, colid, outpos;
     int* h_tile_on = nullptr;
     hipGraphExec_t exec = nullptr;
     int exec_iters = 0;
@@ -321,6 +323,7 @@ void ensure_ws(eqd_solver* s, int K, int ntiles, int prec) {
     if (prec == EQD_PREC_MIXED) w->xc.alloc(w->nvec * sizeof(float));
     w->fields.alloc(slots * 2 * op.g.n23 * op.g.n23 * 8);
     w->partial.alloc((size_t)ntiles * op.G * K * 8);
+    if (op.nfold) w->folded.alloc((size_t)ntiles * op.nfold * K * 8);
     w->cs.alloc(slots * sizeof(ColState));
     w->tile_on.alloc(ntiles * sizeof(int));
     w->tile_rel.alloc(ntiles * sizeof(int));
@@ -346,6 +349,14 @@ struct Engine {
         int* on = w.tile_on.as<int>();
         int* rel = w.tile_rel.as<int>();
         double* part = w.partial.as<double>();
+        // the partials k_finalize reads: the groups' own, or their first-level block totals
+        const double* fpart = op.nfold ? w.folded.as<double>() : part;
+        const int fG = op.nfold ? op.nfold : op.G;
+        auto fold = [&] {
+            if (op.nfold)
+                hipLaunchKernelGGL((k_fold_partials<K>), dim3(op.nfold, w.ntiles), dim3(K * 16), 0, st, op.G,
+                                   op.fchunk, (const double*)part, w.folded.as<double>());
+        };
         OpView<T> v;
         if constexpr (std::is_same<T, float>::value) v = op.v32; else v = op.v64;
         const bool jac = s->o.precond == EQD_PRECOND_JACOBI;
@@ -373,7 +384,8 @@ struct Engine {
             }
         });
         timed(3, [&] {
-            hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, op.G, 0, (int)mixed, R, cs, part, on, rel,
+            fold();
+            hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, fG, 0, (int)mixed, R, cs, fpart, on, rel,
                                nullptr);
         });
         timed(1, [&] {
@@ -393,7 +405,8 @@ struct Engine {
             }
         });
         timed(3, [&] {
-            hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, op.G, 1, (int)mixed, R, cs, part, on, rel,
+            fold();
+            hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, fG, 1, (int)mixed, R, cs, fpart, on, rel,
                                nullptr);
         });
         timed(2, [&] {
@@ -433,7 +446,8 @@ struct Engine {
                                        w.b.as<double>(), r, cs, rel, part, dinv64);
             });
             timed(3, [&] {
-                hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, op.G, 2, 1, R, cs, part, on, rel,
+                fold();
+                hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, fG, 2, 1, R, cs, fpart, on, rel,
                                    nullptr);
             });
         }
@@ -449,8 +463,12 @@ struct Engine {
                            w.b.as<double>(), w.partial.as<double>(),
                            s->o.precond == EQD_PRECOND_JACOBI ? op.dinv64 : nullptr);
         const int64_t itmax = s->o.itmax > 0 ? s->o.itmax : 2 * op.ntot;
-        hipLaunchKernelGGL((k_init_state<K>), dim3(w.ntiles), dim3(K), 0, st, op.G, w.cs.as<ColState>(),
-                           w.partial.as<double>(), w.colid.as<int64_t>(), s->o.rtol, s->o.atol, itmax,
+        if (op.nfold)
+            hipLaunchKernelGGL((k_fold_partials<K>), dim3(op.nfold, w.ntiles), dim3(K * 16), 0, st, op.G, op.fchunk,
+                               (const double*)w.partial.as<double>(), w.folded.as<double>());
+        hipLaunchKernelGGL((k_init_state<K>), dim3(w.ntiles), dim3(K), 0, st, op.nfold ? op.nfold : op.G,
+                           w.cs.as<ColState>(), op.nfold ? w.folded.as<double>() : w.partial.as<double>(),
+                           w.colid.as<int64_t>(), s->o.rtol, s->o.atol, itmax,
                            w.tile_on.as<int>(), mask);
         HIPCHK(hipGetLastError());
     }
@@ -1181,19 +1199,23 @@ eqd_status eqd_operator_create(const eqd_grid_desc* g, eqd_op** out) {
         op->v32 = op->view<float>();
         op->p64 = op->pview<double>();
         op->p32 = op->pview<float>();
-        // Groups per tile: about 4 FP32 pencils per workgroup (one per wave), between 64 and
-        // 1024. With one pencil per wave the blocks of a tile sweep its pencils in order, so the
-        // pencils in flight on an XCD are one window of neighbours (dz = 50 m: 961 pencils ->
-        // 256 groups; stencil 640 -> 558 us per launch against 64 groups of 4 pencils per wave,
-        // whose strided sweeps keep two tiles' windows in each L2; 512 groups leave half the
-        // waves idle, 602 us). One tile of a large grid fills the device (dz = 25 m: 64,516
-        // pencils -> the 1024 cap, which bounds the per-column partial sums of k_finalize).
-        // Depends on the grid only, never on the batch.
+        // Groups per tile: about 4 FP32 pencils per workgroup (one per wave), at least 64 and a
+        // multiple of 64. With one pencil per wave the blocks of a tile sweep its pencils in
+        // order and are short, so the pencils in flight on an XCD are one window of neighbours
+        // (dz = 50 m: 961 pencils -> 256 groups; stencil 640 -> 558 us per launch against 64
+        // groups of 4 pencils per wave, whose strided sweeps kept two tiles' windows in each L2;
+        // 512 groups leave half the waves idle, 602 us). dz = 25 m: 64,516 pencils -> 16,192
+        // groups; grids above kFoldAbove groups sum their partials in two levels
+        // (k_fold_partials) so that k_finalize stays one short pass. Depends on the grid only,
+        // never on the batch.
         {
             const int np = op->p32.NY * op->p32.NY * op->p32.nseg;
-            int G = kGroups;
-            while (G < kMaxGroups && G * EQD_PENCILS_PER_GROUP < np) G *= 2;
-            op->G = G;
+            const int per = EQD_PENCILS_PER_GROUP * 64;
+            op->G = std::min(kMaxGroups, std::max(kGroups, (np + per - 1) / per * 64));
+            if (op->G > kFoldAbove) {
+                op->nfold = 16 * ((op->G + 16 * 256 - 1) / (16 * 256));
+                op->fchunk = (op->G + op->nfold - 1) / op->nfold;
+            }
         }
         op->rv = RhsView{op->tb.NX, op->tb.NY, g->n23, lo, cnt, op->nf, op->up(op->tb.Jt), op->up(op->tb.Jn),
                          op->v64.Hx, op->v64.Hy, op->tab<double>(op->tb.D1full)};

@@ -13,7 +13,8 @@
 namespace eqd {
 
 constexpr int kGroups = 64;      // groups per tile of small grids (the minimum)
-constexpr int kMaxGroups = 1024;  // groups per tile of the largest grids
+constexpr int kMaxGroups = 32768;  // groups per tile of the largest grids
+constexpr int kFoldAbove = 1024;   // grids with more groups sum their partials in two levels
 #ifndef EQD_CANDIDATE_EVERY
 #define EQD_CANDIDATE_EVERY 4
 #endif
@@ -553,6 +554,31 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ partia
         for (int j = 0; j < 16; ++j) s += v[j];
     }
     return s;
+}
+
+// First level of the two-level partial sum of grids with many groups: block (b, tile) sums
+// the partials of groups [b chunk, (b + 1) chunk) of every column, each of 16 strands taking
+// every 16th group in order and the strands then added in order, into
+// folded[(tile nb + b) K + lane]. k_finalize / k_init_state then sum the nb block totals (nb a
+// multiple of 16). The order is fixed by the grid, like the one-level sum.
+template <int K>
+__global__ __launch_bounds__(K * 16) void k_fold_partials(int G, int chunk, const double* __restrict__ partial,
+                                                          double* __restrict__ folded) {
+    __shared__ double red[16 * K];
+    const int b = blockIdx.x, tile = blockIdx.y, nb = (int)gridDim.x;
+    const int lane = threadIdx.x % K, strand = threadIdx.x / K;
+    const int g0 = b * chunk, g1 = min(G, g0 + chunk);
+    const double* p = partial + (size_t)tile * G * K + lane;
+    double acc = 0.0;
+    for (int g = g0 + strand; g < g1; g += 16) acc += p[(size_t)g * K];
+    red[strand * K + lane] = acc;
+    __syncthreads();
+    if (strand == 0) {
+        double tot = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) tot += red[j * K + lane];
+        folded[((size_t)tile * nb + b) * K + lane] = tot;
+    }
 }
 
 // One block per tile, K threads: per-column scalar steps of Krylov's cg!
