@@ -340,7 +340,7 @@ struct Engine {
         Workspace& w = *s->ws;
         const eqd_op& op = *s->op;
         const bool mixed = std::is_same<T, float>::value;
-        const int R = s->o.reliable_interval > 0 ? s->o.reliable_interval : 32;
+        const int R = s->o.reliable_interval > 0 ? s->o.reliable_interval : 48;
         dim3 grid(s->op->G, w.ntiles);
         T* r = w.r.as<T>();
         T* p = w.p.as<T>();
@@ -1174,7 +1174,7 @@ void eqd_cg_opts_default(eqd_cg_opts* o) {
     o->engine = EQD_ENGINE_BATCHED;
     o->tile_k = 0;
     o->max_cols_in_flight = 0;
-    o->reliable_interval = 32;
+    o->reliable_interval = 48;
     o->graph_iters = 8;
 }
 

@@ -88,7 +88,7 @@ class CGSolver:
     `tile_k`, `max_cols_in_flight`, `reliable_interval` and `graph_iters`."""
 
     def __init__(self, op: _Operator, rtol=1e-10, atol=0.0, itmax=0, precond="none", precision="fp64",
-                 engine="batched", tile_k=0, max_cols_in_flight=0, reliable_interval=32, graph_iters=8,
+                 engine="batched", tile_k=0, max_cols_in_flight=0, reliable_interval=48, graph_iters=8,
                  timing=0, _handle=None):
         if precond not in _PRECOND:
             raise EqdError(_lib.EQD_ERR_ARG, f"precond must be :none or :jacobi, got {precond}")

@@ -26,7 +26,7 @@ class Server:
     request ids and `result(id)` returns (tractions 2nf x ncols, iterations per column)."""
 
     def __init__(self, *, precision="mixed", rtol=1e-10, atol=0.0, hbm_budget_bytes=0, graph_iters=8,
-                 reliable_interval=32):
+                 reliable_interval=48):
         o = _lib.default_opts()
         o.rtol, o.atol = float(rtol), float(atol)
         o.precision = _PREC[precision]

@@ -70,7 +70,7 @@ typedef struct {
     int tile_k;               /* columns interleaved per tile: 1, 8, 16, 32 or 64; 0 = autotuned: the first
                                  build times 16 CG iterations of its own first batch at each width */
     int max_cols_in_flight;   /* columns resident at once (0 = all, bounded by memory) */
-    int reliable_interval;    /* mixed precision: iterations between FP64 residual refreshes (0 = 32) */
+    int reliable_interval;    /* mixed precision: iterations between FP64 residual refreshes (0 = 48) */
     int graph_iters;          /* iterations captured per HIP graph replay (0 = 8) */
     int timing;               /* 1: no graphs, HIP events around every kernel (measurement runs) */
 } eqd_cg_opts;
