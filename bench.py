@@ -127,6 +127,56 @@ def bench_bp8(args):
     return res
 
 
+def host_info():
+    """nproc and the CPU model of the host the CPU baseline ran on."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def bench_lf400(args, lam, mu):
+    """The headline grid (dz = 50 m, L = (800, 400)) in the reference's own Omega_f reading, l_f = 400:
+    289 fault nodes, 578 columns, the configuration of the reference's published CPU build times
+    (201.7 s serial = 2.87 K-columns/s, 94.5 s on 16 threads = 6.12 K-columns/s; Julia on a 12-core
+    workstation, PROGRESS.md:370-376 of the reference). The device build here (mixed precision, the
+    headline's settings) beside the CPU restatement on this host's cores."""
+    g = eqd.bp8_grid_pair(args.dz, args.L_fault, args.L_normal)
+    fe = eqd.FaultElasticity(g, lam, mu, l_f=400.0, precision="mixed", tile_k=args.tile_k,
+                             reliable_interval=args.reliable_interval)
+    nf2 = 2 * eqd.frictional_node_count(fe)
+    K = eqd.fault_stiffness(fe)  # untimed: autotune and first graph capture
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        K = eqd.fault_stiffness(fe)
+        ts.append(time.perf_counter() - t0)
+    t = min(ts)
+    res = {"columns": nf2, "device_K_columns_per_s": round(nf2 / t, 1), "device_build_ms": round(t * 1e3, 2),
+           "published_reference_cpu": {"one_thread": 2.87, "16_threads": 6.12, "unit": "K-columns/s",
+                                       "note": "Julia reference, 12-core workstation, PROGRESS.md:370-376"},
+           "device_vs_published_16_threads": round(nf2 / t / 6.12, 1),
+           "K_self": float(K[(nf2 // 2) // 2, (nf2 // 2) // 2])}
+    if not args.no_cpu:
+        from oracle.oracle import OracleSystem, bp8_geometry
+        o = OracleSystem(bp8_geometry(args.dz, args.L_fault, args.L_normal, 400.0))
+        ncpu = min(os.cpu_count() or 1, 16)
+        t0 = time.perf_counter()
+        Ko = o.fault_stiffness(nthreads=ncpu)
+        tc = time.perf_counter() - t0
+        res["cpu_port"] = {"value": round(nf2 / tc, 2), "threads": ncpu, "columns": nf2}
+        res["K_rel_err_vs_oracle"] = float(np.linalg.norm(K - Ko) / np.linalg.norm(Ko))
+    print(f"bench: l_f=400 reading: {nf2} cols on the device in {t * 1e3:.1f} ms "
+          f"({nf2 / t:.0f} K-cols/s, {nf2 / t / 6.12:.0f}x the published 16-thread Julia build)",
+          file=sys.stderr, flush=True)
+    return res
+
+
 def bench_plain_config(args):
     """The CPU-runnable configuration end to end: BP8-QD-GS at dz = 100 m with l_f = L_fault = 800 m
     (289 fault nodes, 578 columns). As written (L_normal = 400) its n1 = 5 < 9 is rejected by
@@ -371,6 +421,8 @@ def main():
     ap.add_argument("--cpu_cols", type=int, default=1024,
                     help="columns of the CPU oracle sample (about 10 s on 16 host threads)")
     ap.add_argument("--no_cpu", action="store_true")
+    ap.add_argument("--cpu_cols_1core", type=int, default=48,
+                    help="columns of the one-thread CPU oracle sample (about 7 s)")
     ap.add_argument("--no_extras", action="store_true", help="only the timed mixed-precision steps")
     ap.add_argument("--no_bp8", action="store_true", help="skip the BP8 time-loop measurement")
     ap.add_argument("--no_plain", action="store_true", help="skip the dz = 100 m end-to-end configuration")
@@ -501,14 +553,24 @@ def main():
             Kc = o.fault_stiffness_cols(csample, nthreads=ncpu)
             tc = time.perf_counter() - t0
             err_oracle = float(np.linalg.norm(K_mixed[:, csample] - Kc) / np.linalg.norm(Kc))
+            # the same restatement on one thread, on the first columns of the sample
+            c1 = csample[:max(1, min(args.cpu_cols_1core, len(csample)))]
+            t0 = time.perf_counter()
+            o.fault_stiffness_cols(c1, nthreads=1)
+            t1 = time.perf_counter() - t0
             out["cpu_baseline"] = {"value": round(len(csample) / tc, 3), "unit": "K-columns/s", "cores": ncpu,
                                    "kind": "port",
                                    "sample": f"{len(csample)} seeded random columns of the same K build, "
-                                             f"assembled-CSR FP64 CG, one column per thread"}
+                                             f"assembled-CSR FP64 CG, one column per thread",
+                                   "one_core": {"value": round(len(c1) / t1, 3), "columns": len(c1)},
+                                   "host": host_info()}
+            print(f"bench: cpu oracle {len(c1)} cols on 1 thread in {t1:.1f} s", file=sys.stderr, flush=True)
             out["K_rel_err_vs_oracle_sample"] = err_oracle
             print(f"bench: cpu oracle {len(csample)} cols in {tc:.1f} s on {ncpu} threads; "
                   f"mixed K vs oracle {err_oracle:.2e}", file=sys.stderr, flush=True)
     Kbuf.free()
+    if not args.no_extras:
+        out["reference_reading_lf400"] = bench_lf400(args, lam, mu)
     if not args.no_extras and not args.no_bp8:
         out["bp8_30day_gs"] = bench_bp8(args)
     if not args.no_extras and not args.no_bp8 and not args.no_plain:
