@@ -538,6 +538,13 @@ def main():
                            "bytes_per_launch": int(bytes_per_launch)}
         out["kernel_share"] = {n: round(v[0] / tot_ms, 4) for n, v in kt.items()}
         out["survey_algorithmic_GBps"] = round(survey_gbs / 1e9, 1)
+        # the survey's full formula: + 6 DOF 8 B per refinement sweep of a column (here every
+        # column of a tile that the FP64 residual pass covered, from the timing-mode build) and
+        # + 3 DOF 8 B per column (RHS, U, extraction)
+        sweeps = kt["residual64"][2] * 64
+        full_bytes = 10 * dof * 4 * col_iters + 6 * dof * 8 * sweeps + 3 * dof * 8 * ncols
+        out["survey_algorithmic_GBps_full"] = round(full_bytes / (ms_per_step / 1e3) / 1e9, 1)
+        out["refinement_sweeps_per_column"] = round(sweeps / ncols, 2)
         out["plain_fp64_one_rhs"] = {"value": round(plain_value, 2), "unit": "K-columns/s",
                                      "sample_columns": len(sample)}
         out["batched_fp64"] = {"value": round(ncols / t64, 2), "unit": "K-columns/s"}

@@ -237,6 +237,7 @@ struct KTimer {
     std::vector<std::pair<int, size_t>> pending;  // kernel id, index of its begin event
     size_t used = 0;
     int cur_active = 0;
+    int cur_rel = 0;  // tiles the FP64 residual pass (kernel 5) covered in the current iteration
     double ms[kKinds] = {};
     int64_t launches[kKinds] = {}, active[kKinds] = {};
     hipEvent_t next() {
@@ -260,7 +261,7 @@ struct KTimer {
             HIPCHK(hipEventElapsedTime(&t, pool[pr.second], pool[pr.second + 1]));
             ms[pr.first] += t;
             launches[pr.first] += 1;
-            active[pr.first] += cur_active;
+            active[pr.first] += pr.first == 5 ? cur_rel : cur_active;
             tot += t;
         }
         pending.clear();
@@ -492,6 +493,13 @@ struct Engine {
             kt.cur_active = act;
             iteration(s, st, &kt);
             HIPCHK(hipStreamSynchronize(st));
+            // tile_rel keeps the flags that the iteration's FP64 residual pass acted on
+            kt.cur_rel = 0;
+            if (std::is_same<T, float>::value) {
+                std::vector<int> rel(w.ntiles);
+                HIPCHK(hipMemcpy(rel.data(), w.tile_rel.p, w.ntiles * sizeof(int), hipMemcpyDeviceToHost));
+                for (int v : rel) kt.cur_rel += v != 0;
+            }
             ms += kt.flush();
             ++iters;
             tile_iters += act;

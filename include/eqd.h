@@ -141,7 +141,8 @@ eqd_status eqd_solver_last_timing(const eqd_solver* s, double* ms, int64_t* sten
  * stiffness build run with opts.timing = 1. kernel: 0 stencil q = A p, 1 x/r update,
  * 2 p update, 3 per-column finalize, 4 reliable-update accumulate, 5 reliable-update
  * FP64 residual. Returns summed ms, launch count and the sum over launches of the
- * number of tiles with an active column (each tile holds tile_k columns). */
+ * number of tiles with an active column (each tile holds tile_k columns); for kernel 5, of
+ * the tiles whose FP64 residual the pass recomputed. */
 eqd_status eqd_solver_kernel_timing(const eqd_solver* s, int kernel, double* total_ms, int64_t* launches,
                                     int64_t* active_tiles);
 
