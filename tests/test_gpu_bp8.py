@@ -118,7 +118,9 @@ def test_run_100h_matches_oracle(toeplitz_K):
 
 def test_run_30day_published_gs(toeplitz_K):
     """PROGRESS.md:671 (30 days, saveat 300, :toeplitz): peak V 1.10e-6 at 2.22 days, final
-    slip 0.0401 m, peak p(0,0) 13.17 MPa -- and the oracle's trajectory."""
+    slip 0.0401 m, peak p(0,0) 13.17 MPa -- and the oracle's trajectory: V_max(t) within 1e-5 at
+    every save (measured 1.3-2e-6; near day 19 the oracle itself moves by 7e-7 when K is scaled by
+    1 + 1e-15, see tests/test_oracle_bp8.py::test_30day_vmax_conditioning)."""
     print("[bp8 run 30 d toeplitz]", flush=True)
     m, sol, ro = _run_pair(toeplitz_K, 30 * 86400.0, 300.0)
     nf = m.nf
@@ -129,7 +131,7 @@ def test_run_30day_published_gs(toeplitz_K):
     assert f"{vmax[j]:.2e}" == "1.10e-06" and f"{sol.t[j] / 86400:.2f}" == "2.22"
     assert f"{sol.u[-1, ic]:.4f}" == "0.0401"
     assert f"{sol.u[:, 3 * nf + ic].max() / 1e6:.2f}" == "13.17"
-    assert close(vmax, ro["vmax"], 1e-4)
+    assert close(vmax, ro["vmax"], 1e-5)
     assert close(sol.u[:, :2 * nf], ro["u"][:, :2 * nf], 1e-5, 1e-12)
 
 
