@@ -125,11 +125,14 @@ def bench_bp8(args):
         res["vmax_max_rel_diff_vs_oracle"] = float(d.max())
         res["vmax_median_rel_diff_vs_oracle"] = float(np.median(d))
         res["vmax_max_rel_diff_day"] = round(float(sol.t[int(np.argmax(d))]) / 86400, 2)
-        # the trajectory's own conditioning: the oracle against itself with K scaled by 1 + 1e-15
-        rp = OracleBP8(m.K * (1 + 1e-15), m.n2f, dict(dz=50.0)).run(0.0, tf, saveat)
-        dp = np.abs(rp["vmax"] - ro["vmax"]) / ro["vmax"]
-        res["oracle_vs_itself_K_times_1p1e-15"] = {"vmax_max_rel_diff": float(dp.max()),
-                                                   "day": round(float(sol.t[int(np.argmax(dp))]) / 86400, 2)}
+        # the trajectory's own conditioning: the oracle against itself with K scaled by a few ulps
+        worst, day = 0.0, 0.0
+        for scale in (1 + 1e-15, 1 - 1e-15, 1 + 4e-15, 1 - 4e-15):
+            rp = OracleBP8(m.K * scale, m.n2f, dict(dz=50.0)).run(0.0, tf, saveat)
+            dp = np.abs(rp["vmax"] - ro["vmax"]) / ro["vmax"]
+            if dp.max() > worst:
+                worst, day = float(dp.max()), round(float(sol.t[int(np.argmax(dp))]) / 86400, 2)
+        res["oracle_vs_itself_K_scaled_1pm1e-15_4e-15"] = {"vmax_max_rel_diff": worst, "day": day}
         print(f"bench: bp8 30 d cpu oracle {t_cpu * 1e3:.0f} ms ({ro['naccept']} steps)", file=sys.stderr,
               flush=True)
     return res
