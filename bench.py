@@ -308,6 +308,8 @@ def bench_ensemble(args):
     t0 = time.perf_counter()
     st = e.run(tspan=(0.0, args.ens_hours * 3600.0), saveat=3600.0)
     t_run = time.perf_counter() - t0
+    print(f"bench: ensemble GS {args.ens_members} members {args.ens_hours} h: loop {e.loop_ms:.0f} ms",
+          file=sys.stderr, flush=True)
     cpu_check = None
     if not getattr(args, "no_cpu", True):
         # member-wise against the CPU oracle at full size: members 0 (base) and 1 over the first
@@ -326,15 +328,37 @@ def bench_ensemble(args):
     ms_gemm, flops = e.gemm_timing(20)
     tf = flops / (ms_gemm * 1e-3)
     acc = [s.naccept for s in st]
+    ens_loop_ms = e.loop_ms
+    del e
+    # the same members with the Peaceman well (BP8-QD-PW) over a shorter horizon: its steps grow
+    # as the well pressure builds (104 in the first 3 h, ~1e6 per 100 h at this spacing by the
+    # reference's dz^-4 scaling of its 31,249 steps per 100 h at dz = 50 m)
+    pw = None
+    if args.ens_pw_hours > 0:
+        mpw = bp8.model_from_stiffness(Kh, m.n2f, par=m.par, injection="peaceman")
+        epw = Ensemble(mpw, args.ens_members, **mem)
+        print(f"bench: ensemble PW {args.ens_members} members, {args.ens_pw_hours} h ...", file=sys.stderr,
+              flush=True)
+        t0 = time.perf_counter()
+        stp = epw.run(tspan=(0.0, args.ens_pw_hours * 3600.0), saveat=600.0)
+        t_pw = time.perf_counter() - t0
+        accp = [s.naccept for s in stp]
+        pw = {"hours": args.ens_pw_hours, "run_ms": round(t_pw * 1e3, 1), "device_loop_ms": round(epw.loop_ms, 1),
+              "accepted_steps_min_max": [int(min(accp)), int(max(accp))],
+              "member_steps_per_s": round(sum(accp) / (epw.loop_ms * 1e-3), 1),
+              "retcodes_ok": bool(all(s.retcode == 0 for s in stp))}
+        print(f"bench: ensemble PW {args.ens_members} members {args.ens_pw_hours} h: loop {epw.loop_ms:.0f} ms, "
+              f"steps {min(accp)}-{max(accp)}", file=sys.stderr, flush=True)
+        del epw, mpw
     res = {"nf": m.nf, "members": args.ens_members, "hours": args.ens_hours, "K_build_ms": round(t_build * 1e3, 1),
-           "run_ms": round(t_run * 1e3, 1), "device_loop_ms": round(e.loop_ms, 1),
+           "run_ms": round(t_run * 1e3, 1), "device_loop_ms": round(ens_loop_ms, 1),
            "accepted_steps_min_max": [int(min(acc)), int(max(acc))],
-           "member_steps_per_s": round(sum(acc) / (e.loop_ms * 1e-3), 1), "single_trajectory": single,
-           "cpu_member_check": cpu_check,
+           "member_steps_per_s": round(sum(acc) / (ens_loop_ms * 1e-3), 1), "single_trajectory": single,
+           "cpu_member_check": cpu_check, "peaceman": pw,
            "gemm": {"bound": "mfma", "achieved": round(tf / 1e12, 2), "peak": 78.6, "unit": "TFLOP/s",
                     "frac": round(tf / 78.6e12, 4), "ms_per_gemm": round(ms_gemm, 3), "flop_per_gemm": flops}}
     print(f"bench: ensemble dz=20 {args.ens_members} members {args.ens_hours} h: K build {t_build * 1e3:.0f} ms, "
-          f"loop {e.loop_ms:.0f} ms, GEMM {ms_gemm:.3f} ms = {tf / 1e12:.1f} TFLOP/s", file=sys.stderr, flush=True)
+          f"loop {ens_loop_ms:.0f} ms, GEMM {ms_gemm:.3f} ms = {tf / 1e12:.1f} TFLOP/s", file=sys.stderr, flush=True)
     return res
 
 
@@ -443,6 +467,8 @@ def main():
     ap.add_argument("--no_ensemble", action="store_true", help="skip the dz = 20 m trajectory ensemble")
     ap.add_argument("--ens_members", type=int, default=256)
     ap.add_argument("--ens_hours", type=float, default=720.0, help="ensemble horizon (30 days)")
+    ap.add_argument("--ens_pw_hours", type=float, default=3.0,
+                    help="horizon of the Peaceman-well ensemble (0: skip; its steps grow fast with the horizon)")
     ap.add_argument("--serve_seconds", type=float, default=10.0)
     ap.add_argument("--serve_rate", type=float, default=10.0,
                     help="mean request rate; 50 req/s over this grid mix exceeds one GPU (see DESIGN.md)")
