@@ -461,6 +461,7 @@ def main():
     ap.add_argument("--no_extras", action="store_true", help="only the timed mixed-precision steps")
     ap.add_argument("--no_bp8", action="store_true", help="skip the BP8 time-loop measurement")
     ap.add_argument("--no_plain", action="store_true", help="skip the dz = 100 m end-to-end configuration")
+    ap.add_argument("--no_lf400", action="store_true", help="skip the l_f = 400 reading of the headline grid")
     ap.add_argument("--no_serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no_production", action="store_true", help="skip the dz = 25 m production-size build")
     ap.add_argument("--prod_cols", type=int, default=512, help="exact columns of the dz = 25 m subset")
@@ -613,7 +614,7 @@ def main():
             print(f"bench: cpu oracle {len(csample)} cols in {tc:.1f} s on {ncpu} threads; "
                   f"mixed K vs oracle {err_oracle:.2e}", file=sys.stderr, flush=True)
     Kbuf.free()
-    if not args.no_extras:
+    if not args.no_extras and not args.no_lf400:
         out["reference_reading_lf400"] = bench_lf400(args, lam, mu)
     if not args.no_extras and not args.no_bp8:
         out["bp8_30day_gs"] = bench_bp8(args)
