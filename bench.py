@@ -378,7 +378,8 @@ def bench_serving(args):
     gids, axes = [], []
     t0 = time.perf_counter()
     for dz, Lf, Ln in specs:
-        gids.append(sv.add_grid(eqd.bp8_grid_pair(dz, Lf, Ln), lam, mu, l_f=Lf, max_columns=args.serve_lanes))
+        lanes = args.serve_lanes_dz25 if dz == 25.0 else args.serve_lanes
+        gids.append(sv.add_grid(eqd.bp8_grid_pair(dz, Lf, Ln), lam, mu, l_f=Lf, max_columns=lanes))
         n2f = sv.grids[-1]["n2f"]
         axes.append(np.linspace(-Lf, Lf, n2f))
     t_setup = time.perf_counter() - t0
@@ -474,6 +475,9 @@ def main():
     ap.add_argument("--serve_rate", type=float, default=10.0,
                     help="mean request rate; 50 req/s over this grid mix exceeds one GPU (see DESIGN.md)")
     ap.add_argument("--serve_lanes", type=int, default=512, help="lanes per grid in the serving pools")
+    ap.add_argument("--serve_lanes_dz25", type=int, default=64,
+                    help="lanes of the dz = 25 m pool: one tile already fills the device there, so a larger "
+                         "pool adds latency and no throughput")
     ap.add_argument("--copy_calib_mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (3 times) and exit: the "
                          "reference kernel that scales the FETCH_SIZE / WRITE_SIZE counters")

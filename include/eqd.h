@@ -290,7 +290,10 @@ typedef struct eqd_server eqd_server;
 eqd_status eqd_server_create(const eqd_cg_opts* opts, size_t hbm_budget_bytes, eqd_server** out);
 eqd_status eqd_server_destroy(eqd_server* sv);
 /* Makes a grid resident: operator tables, a pool of 64-column tiles (as many as the
- * remaining budget holds, at most ceil(max_columns / 64) when max_columns > 0). */
+ * remaining budget holds, at most ceil(max_columns / 64) when max_columns > 0). On a grid
+ * where one tile's launches already fill the device (dz = 25 m, L = (1600, 800)), the pool's
+ * columns/s does not grow with its size while every column's latency does (each iteration
+ * covers every active tile): give such a grid the lanes its largest request needs. */
 eqd_status eqd_server_add_grid(eqd_server* sv, const eqd_grid_desc* g, int max_columns, int* grid_id, int* slots);
 /* Queues a request for the unit-slip columns cols[0..ncols) (0-based, < 2nf) of K, i.e.
  * fault_stiffness restricted to those columns (src/FaultResponse.jl:229-287). */
