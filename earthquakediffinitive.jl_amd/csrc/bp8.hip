@@ -434,44 +434,70 @@ __global__ void k_transpose(const double* __restrict__ K, long long ldK, int n, 
 // K s of a Toeplitz K (src/FaultResponse.jl:378-390) from its separation kernel: K's four nf x nf
 // blocks depend on the node separation only, so the contribution of source x3 row bj to target
 // x3 row ib is a 1D correlation of the kernel row (ib - bj) with the slip row bj. Block
-// (ib, chunk) stages that kernel row (2 n2f - 1 entries of 4 blocks) and the slip row in LDS for
-// every bj of its chunk and leaves its partial sums of target row ib in ks[chunk][node][2]; the
-// stage kernel adds the chunks in order. No row of K is read (at nf = 6561: 41 MB of L2 reads
-// per evaluation instead of 2.8 GB).
-__global__ __launch_bounds__(128) void k_toeplitz_ks(Dev d, int stage, const double* __restrict__ y) {
+// (ib, chunk) stages those kernel rows (2 n2f - 1 entries of 4 blocks) and slip rows in LDS, R
+// rows of its chunk at a time with one thread per (row slot, target node), and leaves its partial
+// sums of target row ib in ks[chunk][node][2]; the stage kernel adds the chunks in order. No row
+// of K is read (at nf = 6561: 41 MB of L2 reads per evaluation instead of 2.8 GB).
+//
+// Threads per block of k_toeplitz_ks, and the source rows a block works on at once: as many
+// as its threads cover with one thread per target node of the row.
+constexpr int kKsThreads = 256;
+__host__ __device__ inline int ks_slots(int n2) { return n2 <= kKsThreads ? kKsThreads / n2 : 1; }
+
+__global__ __launch_bounds__(kKsThreads) void k_toeplitz_ks(Dev d, int stage, const double* __restrict__ y) {
     const Consts& c = d.c;
     if (stage != 0 && d.ctl->done) return;
     extern __shared__ __attribute__((aligned(32))) double ks_lds[];
     const int n2 = c.n2f, W2 = 2 * n2 - 1, nf = c.nf;
-    double4* Trow = reinterpret_cast<double4*>(ks_lds);     // W2 entries
-    double2* srow = reinterpret_cast<double2*>(Trow + W2);   // n2 entries (s2, s3)
+    // R source rows per round (row slot r = thread / T), T targets per pass
+    const int R = ks_slots(n2), T = n2 <= kKsThreads ? n2 : kKsThreads;
+    double4* Trow = reinterpret_cast<double4*>(ks_lds);                 // R x W2
+    double2* srow = reinterpret_cast<double2*>(Trow + (size_t)R * W2);   // R x n2
+    double2* red = reinterpret_cast<double2*>(ks_lds);                   // R x T, after the rounds
     const int ib = blockIdx.x, chunk = blockIdx.y;
     const int per = (n2 + d.ks_split - 1) / d.ks_split;
     const int b0 = chunk * per, b1 = min(n2, b0 + per);
-    const double4* T = reinterpret_cast<const double4*>(d.Tk);
-    for (int ia0 = 0; ia0 < n2; ia0 += blockDim.x) {
-        const int ia = ia0 + threadIdx.x;
+    const double4* Tk = reinterpret_cast<const double4*>(d.Tk);
+    const int t = threadIdx.x, r = t / T, it = t - r * T;
+    for (int ia0 = 0; ia0 < n2; ia0 += T) {
+        const int ia = ia0 + it;
         double a2 = 0.0, a3 = 0.0;
-        for (int bj = b0; bj < b1; ++bj) {
-            __syncthreads();  // the previous row's readers are done
-            const size_t trow = (size_t)W2 * (ib - bj + n2 - 1);
-            for (int q = threadIdx.x; q < W2; q += blockDim.x) Trow[q] = T[trow + q];
-            for (int q = threadIdx.x; q < n2; q += blockDim.x)
-                srow[q] = make_double2(y[q + (size_t)n2 * bj], y[nf + q + (size_t)n2 * bj]);
+        for (int rb = b0; rb < b1; rb += R) {
+            __syncthreads();  // the previous round's readers are done
+            for (int q = t; q < R * W2; q += blockDim.x) {
+                const int sr = q / W2, bj = rb + sr;
+                if (bj < b1) Trow[q] = Tk[(size_t)W2 * (ib - bj + n2 - 1) + (q - sr * W2)];
+            }
+            for (int q = t; q < R * n2; q += blockDim.x) {
+                const int sr = q / n2, bj = rb + sr, aj = q - sr * n2;
+                if (bj < b1) srow[q] = make_double2(y[aj + (size_t)n2 * bj], y[nf + aj + (size_t)n2 * bj]);
+            }
             __syncthreads();
-            if (ia < n2) {
+            if (r < R && rb + r < b1 && ia < n2) {
+                const double4* Tr = Trow + (size_t)r * W2 + ia + n2 - 1;
+                const double2* sr = srow + (size_t)r * n2;
                 for (int aj = 0; aj < n2; ++aj) {
-                    const double4 k = Trow[ia - aj + n2 - 1];
-                    const double2 sv = srow[aj];
+                    const double4 k = Tr[-aj];
+                    const double2 sv = sr[aj];
                     a2 += k.x * sv.x + k.z * sv.y;
                     a3 += k.y * sv.x + k.w * sv.y;
                 }
             }
         }
-        if (ia < n2) {
+        // the row slots' sums, added in slot order
+        __syncthreads();
+        if (r < R) red[t] = make_double2(a2, a3);
+        __syncthreads();
+        if (r == 0 && ia < n2) {
+            double s2 = 0.0, s3 = 0.0;
+            for (int q = 0; q < R; ++q) {
+                const double2 v = red[q * T + it];
+                s2 += v.x;
+                s3 += v.y;
+            }
             const size_t o = ((size_t)chunk * nf + ia + (size_t)n2 * ib) * 2;
-            d.ks[o] = a2;
-            d.ks[o + 1] = a3;
+            d.ks[o] = s2;
+            d.ks[o + 1] = s3;
         }
     }
 }
@@ -1187,9 +1213,9 @@ void initial_state(const eqd_bp8& m, double* u) {
 // K s partials of a Toeplitz-K model for the stage about to run on y (no-op for a dense K).
 void toeplitz_ks(eqd_bp8& m, const Dev& d, int stage, const double* y) {
     if (!m.ks_split) return;
-    const int n2 = m.c.n2f;
-    const size_t lds = (size_t)(2 * n2 - 1) * 32 + (size_t)n2 * 16;
-    hipLaunchKernelGGL(k_toeplitz_ks, dim3(n2, m.ks_split), dim3(128), lds, m.st, d, stage, y);
+    const int n2 = m.c.n2f, R = ks_slots(n2);
+    const size_t lds = (size_t)R * (2 * n2 - 1) * 32 + (size_t)R * n2 * 16;
+    hipLaunchKernelGGL(k_toeplitz_ks, dim3(n2, m.ks_split), dim3(kKsThreads), lds, m.st, d, stage, y);
 }
 
 // One rhs! at a device state into a device output, then the Newton status.
