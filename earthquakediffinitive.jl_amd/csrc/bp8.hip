@@ -287,6 +287,107 @@ __global__ __launch_bounds__(256) void k_bp8_stage(Dev d, int stage, const doubl
     }
 }
 
+// The stage of a model whose K s comes precomputed per node from k_toeplitz_ks (ks_split > 0):
+// one node per thread, where the wave-per-node k_bp8_stage would repeat each node's scalar
+// Newton on 64 lanes. The per-node arithmetic is k_bp8_stage's except the order of two sums
+// (the K s chunks and the A_p row are added here one after another instead of across lanes).
+__global__ __launch_bounds__(256) void k_bp8_stage_nodes(Dev d, int stage, const double* __restrict__ y,
+                                                         double* __restrict__ kout, double* __restrict__ ynext,
+                                                         double t_fixed) {
+    const Consts& c = d.c;
+    if (stage != 0 && d.ctl->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.nf) return;
+    const double dt = stage != 0 ? d.ctl->dt : 0.0;
+    const double t = stage != 0 ? d.ctl->t + kC[stage] * dt : t_fixed;
+
+    double dtau2 = 0.0, dtau3 = 0.0;
+    for (int q = 0; q < d.ks_split; ++q) {
+        dtau2 += d.ks[((size_t)q * c.nf + i) * 2];
+        dtau3 += d.ks[((size_t)q * c.nf + i) * 2 + 1];
+    }
+    const int n = c.n2f, ia = i % n, ib = i / n;
+    const double* P = y + 3 * c.nf;
+    double ap = 0.0;
+    {
+        const int sa = band_start(ia, n, c.lw), sb = band_start(ib, n, c.lw);
+        for (int l = 0; l < c.lw; ++l)
+            ap += d.Lx[ia * c.lw + l] * P[sa + l + n * ib] + d.Lx[ib * c.lw + l] * P[ia + n * (sb + l)];
+    }
+    ap = c.alpha * ap;
+
+    const double phi = y[2 * c.nf + i], p = P[i];
+    double v2 = 0.0, v3 = 0.0, vm = 0.0, t2, t3;
+    const double T2 = c.tau0_2 + dtau2, T3 = c.tau0_3 + dtau3;
+    if (!d.active[i]) {
+        t2 = T2;
+        t3 = T3;
+    } else {
+        const double sraw = c.sigma0 - p;
+        const double sig = fmax(sraw, c.sigma_min);
+        const double theta = exp(phi);
+        const double T = sqrt(T2 * T2 + T3 * T3);
+        if (T != 0.0) {
+            double V;
+            if (!slip_rate(T, theta, sig, c, d.Vprev[i], V)) d.fail[0] = 1;
+            v2 = V * T2 / T;
+            v3 = V * T3 / T;
+        }
+        vm = sqrt(v2 * v2 + v3 * v3);
+        t2 = T2 - c.eta * v2;
+        t3 = T3 - c.eta * v3;
+        d.Vprev[i] = fmax(vm, 1e-30);
+        if (sraw < d.sig_low[i]) d.sig_low[i] = sraw;
+        if (sraw < c.sigma_min) {
+            d.floor_hits[i] += 1;
+            d.floor_mask[i] = 1;
+        }
+    }
+    double f[5];
+    f[0] = v2;
+    f[1] = v3;
+    f[2] = exp(-phi) - vm / c.D_RS;
+    f[3] = ap;
+    f[4] = 0.0;
+    int ne = 4;
+    const int idx[5] = {i, c.nf + i, 2 * c.nf + i, 3 * c.nf + i, 4 * c.nf};
+    if (c.injection == 0) {
+        f[3] += (t < c.t_off ? c.src_rate : 0.0) * d.src[i];
+    } else if (i == c.well_cell) {
+        const double pw = y[4 * c.nf];
+        const double transfer = c.WI * (pw - p);
+        f[3] += transfer / c.S_e;
+        f[4] = ((t < c.t_off ? c.Q0 : 0.0) - transfer) / c.S_well;
+        ne = 5;
+    }
+    d.V2[i] = v2;
+    d.V3[i] = v3;
+    d.Vm[i] = vm;
+    d.tau2[i] = t2;
+    d.tau3[i] = t3;
+    double tot = 0.0;  // stage 7: the node's squared scaled error estimates, in entry order
+    for (int q = 0; q < ne; ++q) {
+        const int e = idx[q];
+        const double fe = f[q];
+        kout[e] = fe;
+        if (stage >= 2 && stage <= 6) {
+            double s = 0.0;
+            for (int j = 0; j < stage - 1; ++j) s += kA[stage + 1][j] * d.k[(size_t)j * c.nstate + e];
+            s += kA[stage + 1][stage - 1] * fe;
+            ynext[e] = d.uprev[e] + dt * s;
+        } else if (stage == 7) {
+            double s = 0.0;
+            for (int j = 0; j < 6; ++j) s += kBt[j] * d.k[(size_t)j * c.nstate + e];
+            s += kBt[6] * fe;
+            const double ut = dt * s;
+            const double sc = abstol_of(c, e) + fmax(fabs(d.uprev[e]), fabs(d.unew[e])) * d.ctl->reltol;
+            const double r = ut / sc;
+            tot += r * r;
+        }
+    }
+    if (stage == 7) d.partial[i] = tot;
+}
+
 // Controller (one block): RMS error norm, PI step control with the OrdinaryDiffEq defaults,
 // accept/reject, saveat bookkeeping and the next step size clipped to the next stop.
 __global__ __launch_bounds__(256) void k_bp8_control(Dev d) {
@@ -1218,12 +1319,20 @@ void toeplitz_ks(eqd_bp8& m, const Dev& d, int stage, const double* y) {
     hipLaunchKernelGGL(k_toeplitz_ks, dim3(n2, m.ks_split), dim3(kKsThreads), lds, m.st, d, stage, y);
 }
 
+// One stage kernel: a thread per node when K s comes precomputed, else a wave per node.
+void stage_launch(eqd_bp8& m, const Dev& d, int stage, const double* y, double* kout, double* ynext, double t) {
+    if (m.ks_split)
+        hipLaunchKernelGGL(k_bp8_stage_nodes, dim3((m.c.nf + 255) / 256), dim3(256), 0, m.st, d, stage, y, kout,
+                           ynext, t);
+    else
+        hipLaunchKernelGGL(k_bp8_stage, dim3(m.node_blocks()), dim3(256), 0, m.st, d, stage, y, kout, ynext, t);
+}
+
 // One rhs! at a device state into a device output, then the Newton status.
 void rhs_device(eqd_bp8& m, const double* y_dev, double t, double* out_dev) {
     Dev d = m.dev();
     toeplitz_ks(m, d, 0, y_dev);
-    hipLaunchKernelGGL(k_bp8_stage, dim3(m.node_blocks()), dim3(256), 0, m.st, d, 0, y_dev, out_dev,
-                       (double*)nullptr, t);
+    stage_launch(m, d, 0, y_dev, out_dev, (double*)nullptr, t);
     HIPCHK(hipGetLastError());
 }
 
@@ -1260,8 +1369,7 @@ void build_graph(eqd_bp8& m) {
         double* out[8] = {nullptr, nullptr, yb, ya, yb, ya, un, nullptr};
         for (int stage = 2; stage <= 7; ++stage) {
             toeplitz_ks(m, d, stage, in[stage]);
-            hipLaunchKernelGGL(k_bp8_stage, dim3(m.node_blocks()), dim3(256), 0, m.st, d, stage, in[stage],
-                               k + (size_t)(stage - 1) * ns, out[stage], 0.0);
+            stage_launch(m, d, stage, in[stage], k + (size_t)(stage - 1) * ns, out[stage], 0.0);
         }
         hipLaunchKernelGGL(k_bp8_control, dim3(1), dim3(256), 0, m.st, d);
         hipLaunchKernelGGL(k_bp8_commit, dim3(m.state_blocks()), dim3(256), 0, m.st, d, ya, 0);
