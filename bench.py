@@ -572,7 +572,8 @@ def main():
         # full-line pencils (one 16-node segment covers x1: 2 n1 - 3 <= 16)
         full = "true" if 2 * g.n1 - 3 <= 16 else "false"
         tk = eqd.solver_report(fet.rs)["tile_k"]
-        stencil_name = (f"k_pencil<float,float,0,{full}>" if tk == 64 else f"k_apply_dot<float,{tk}>")
+        w15 = ",15" if 2 * g.n1 - 3 <= 15 else ""  # x1 lines of at most 15 nodes use the 15-wide pencil
+        stencil_name = (f"k_pencil<float,float,0,{full}{w15}>" if tk == 64 else f"k_apply_dot<float,{tk}>")
         out["roofline"] = {"bound": "hbm", "kernel": stencil_name,
                            "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK, 4), "traffic": measured_traffic(stencil_name),

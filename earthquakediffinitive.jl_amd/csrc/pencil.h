@@ -315,12 +315,14 @@ __device__ __forceinline__ void pencil_apply(const PencilView<T>& op, const XBan
 // MODE 1: reliable-update residual r = b - A x (FP64 in, r stored as R for flagged
 //         lanes) with the weighted partial r.r of the FP64 residual (r.(dinv r) when a
 //         Jacobi dinv is given).
-template <typename T, typename R, int MODE, bool F>
+// SW: pencil width (default: pencil_s<T>()); a full-line pencil of an x1 line shorter than that
+// can be instantiated narrower so that no load falls past the line's end.
+template <typename T, typename R, int MODE, bool F, int SW = pencil_s<T>()>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(F ? EQD_PENCIL_WAVES : EQD_PENCIL_WAVES - 1))) void k_pencil(
     PencilView<T> op, int NN, const T* __restrict__ in, T* __restrict__ q, const T* __restrict__ b,
     R* __restrict__ rout, const ColState* __restrict__ cs, const int* __restrict__ tile_on,
     double* __restrict__ partial, const double* __restrict__ dinv) {
-    constexpr int S = pencil_s<T>();
+    constexpr int S = SW;
     int tile, g;
     tile_group(tile, g);
     // A block of an inactive tile leaves before staging anything: k_finalize reads the
