@@ -392,7 +392,7 @@ struct Engine {
         });
         timed(3, [&] {
             fold();
-            hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, fG, 0, (int)mixed, R, cs, fpart, on, rel,
+            hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K * kFinStrands), 0, st, fG, 0, (int)mixed, R, cs, fpart, on, rel,
                                nullptr);
         });
         timed(1, [&] {
@@ -413,7 +413,7 @@ struct Engine {
         });
         timed(3, [&] {
             fold();
-            hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, fG, 1, (int)mixed, R, cs, fpart, on, rel,
+            hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K * kFinStrands), 0, st, fG, 1, (int)mixed, R, cs, fpart, on, rel,
                                nullptr);
         });
         timed(2, [&] {
@@ -454,7 +454,7 @@ struct Engine {
             });
             timed(3, [&] {
                 fold();
-                hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K), 0, st, fG, 2, 1, R, cs, fpart, on, rel,
+                hipLaunchKernelGGL((k_finalize<K>), dim3(w.ntiles), dim3(K * kFinStrands), 0, st, fG, 2, 1, R, cs, fpart, on, rel,
                                    nullptr);
             });
         }

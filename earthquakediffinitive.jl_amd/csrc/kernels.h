@@ -586,62 +586,104 @@ __global__ __launch_bounds__(K * 16) void k_fold_partials(int G, int chunk, cons
 // mode 0: after the stencil (alpha, zero-curvature exit)
 // mode 1: after the update (convergence test, beta, iteration count, itmax)
 // mode 2: after a reliable-update residual (mixed precision only)
+// Sum of a column's partials of groups [g0, g1) in group order, 16 loads in flight per round.
 template <int K>
-__global__ void k_finalize(int G, int mode, int mixed, int rel_interval, ColState* __restrict__ cs,
-                           const double* __restrict__ partial, int* __restrict__ tile_on,
-                           int* __restrict__ tile_rel, int* __restrict__ active_count) {
-    const int tile = blockIdx.x, lane = threadIdx.x;
+__device__ __forceinline__ double sum_partials_range(const double* __restrict__ partial, int G, int tile, int lane,
+                                                     int g0, int g1) {
+    const double* p = partial + (size_t)tile * G * K + lane;
+    double s = 0.0;
+    int g = g0;
+    for (; g + 16 <= g1; g += 16) {
+        double v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = p[(size_t)(g + j) * K];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) s += v[j];
+    }
+    for (; g < g1; ++g) s += p[(size_t)g * K];
+    return s;
+}
+
+// Strands of k_finalize: each sums a contiguous quarter of a column's group partials, the four
+// sums are added in strand order, so the block's load rounds are a quarter of a one-strand sum.
+constexpr int kFinStrands = 4;
+
+// One block per tile, K x kFinStrands threads: per-column scalar steps of Krylov's cg!
+// (src/ElasticitySplitNode.jl:379 -> Krylov 0.10.9) and the tile activity flags, the column's
+// steps on strand 0.
+// mode 0: after the stencil (alpha, zero-curvature exit)
+// mode 1: after the update (convergence test, beta, iteration count, itmax)
+// mode 2: after a reliable-update residual (mixed precision only)
+template <int K>
+__global__ __launch_bounds__(K * kFinStrands) void k_finalize(int G, int mode, int mixed, int rel_interval,
+                                                              ColState* __restrict__ cs,
+                                                              const double* __restrict__ partial,
+                                                              int* __restrict__ tile_on, int* __restrict__ tile_rel,
+                                                              int* __restrict__ active_count) {
+    __shared__ double red[kFinStrands * K];
+    const int tile = blockIdx.x, lane = threadIdx.x % K, strand = threadIdx.x / K;
     ColState& st = cs[(size_t)tile * K + lane];
+    const bool active = st.status == kActive;  // read before strand 0 changes the state
+    {
+        const int per = (G + kFinStrands - 1) / kFinStrands;
+        const int g0 = min(G, strand * per), g1 = min(G, g0 + per);
+        red[strand * K + lane] = active ? sum_partials_range<K>(partial, G, tile, lane, g0, g1) : 0.0;
+    }
+    __syncthreads();
     const double meps = 2.220446049250313e-16;
-    if (mode == 0) st.xpend = 0;  // the previous iteration's p update has run
-    if (st.status == kActive) {
-        const double s = sum_partials<K>(partial, G, tile, lane);
-        if (mode == 0) {
-            if (s <= meps * st.pnorm2 && fabs(s) <= meps * st.pnorm2) {
-                st.status = kZeroCurv;
+    if (strand == 0) {
+        if (mode == 0) st.xpend = 0;  // the previous iteration's p update has run
+        if (active) {
+            double s = red[lane];
+#pragma unroll
+            for (int j = 1; j < kFinStrands; ++j) s += red[j * K + lane];
+            if (mode == 0) {
+                if (s <= meps * st.pnorm2 && fabs(s) <= meps * st.pnorm2) {
+                    st.status = kZeroCurv;
+                } else {
+                    st.alpha = st.rr / s;
+                }
+            } else if (mode == 1) {
+                st.xpend = 1;  // updated this iteration, whatever the convergence test says
+                const double rnorm = sqrt(s);
+                const bool solved = rnorm <= st.eps || rnorm + 1.0 <= 1.0;
+                st.iters += 1;
+                st.since_rel += 1;
+                st.rel_flag = 0;
+                if (solved && !mixed) {
+                    st.status = kSolved;
+                } else {
+                    const double beta = s / st.rr;
+                    st.beta = beta;
+                    st.pnorm2 = s + beta * beta * st.pnorm2;
+                    st.rr = s;
+                    // A column whose FP32 residual claims convergence is checked in FP64 only on
+                    // iterations divisible by kCandidateEvery: the columns of a tile then share
+                    // their checks instead of each triggering a full-tile FP64 pass. The schedule
+                    // depends on the column's own iteration count only.
+                    if (mixed && ((solved && st.iters % kCandidateEvery == 0) || st.since_rel >= rel_interval))
+                        st.rel_flag = 1;
+                    if (st.iters >= st.itmax) {
+                        st.status = kTired;
+                        st.rel_flag = mixed ? 1 : 0;  // fold the last correction into x
+                    }
+                }
             } else {
-                st.alpha = st.rr / s;
-            }
-        } else if (mode == 1) {
-            st.xpend = 1;  // updated this iteration, whatever the convergence test says
-            const double rnorm = sqrt(s);
-            const bool solved = rnorm <= st.eps || rnorm + 1.0 <= 1.0;
-            st.iters += 1;
-            st.since_rel += 1;
-            st.rel_flag = 0;
-            if (solved && !mixed) {
-                st.status = kSolved;
-            } else {
-                const double beta = s / st.rr;
-                st.beta = beta;
-                st.pnorm2 = s + beta * beta * st.pnorm2;
-                st.rr = s;
-                // A column whose FP32 residual claims convergence is checked in FP64 only on
-                // iterations divisible by kCandidateEvery: the columns of a tile then share
-                // their checks instead of each triggering a full-tile FP64 pass. The schedule
-                // depends on the column's own iteration count only.
-                if (mixed && ((solved && st.iters % kCandidateEvery == 0) || st.since_rel >= rel_interval))
-                    st.rel_flag = 1;
-                if (st.iters >= st.itmax) {
-                    st.status = kTired;
-                    st.rel_flag = mixed ? 1 : 0;  // fold the last correction into x
+                if (st.rel_flag) {
+                    const double rnorm = sqrt(s);
+                    st.since_rel = 0;
+                    if (rnorm <= st.eps || rnorm + 1.0 <= 1.0) st.status = kSolved;
+                    else st.rr = s;
                 }
             }
-        } else {
-            if (st.rel_flag) {
-                const double rnorm = sqrt(s);
-                st.since_rel = 0;
-                if (rnorm <= st.eps || rnorm + 1.0 <= 1.0) st.status = kSolved;
-                else st.rr = s;
-            }
+        } else if (mode == 2 && st.rel_flag) {
+            st.since_rel = 0;  // a column that hit itmax in this iteration: its correction was folded
         }
-    } else if (mode == 2 && st.rel_flag) {
-        st.since_rel = 0;  // a column that hit itmax in this iteration: its correction was folded
+        if (mode == 2) st.rel_flag = 0;
     }
-    if (mode == 2) st.rel_flag = 0;
-    const int on = __syncthreads_or(st.status == kActive);
-    const int rel = __syncthreads_or(st.rel_flag != 0);
-    if (lane == 0) {
+    const int on = __syncthreads_or(strand == 0 && st.status == kActive);
+    const int rel = __syncthreads_or(strand == 0 && st.rel_flag != 0);
+    if (threadIdx.x == 0) {
         tile_on[tile] = on;
         if (mode == 1) tile_rel[tile] = rel;
         if (active_count && on) atomicAdd(active_count, 1);
