@@ -109,6 +109,10 @@ struct eqd_op {
     }
     PencilView<double> p64{};
     PencilView<float> p32{};
+    // x1 lines of 15 nodes (n1 = 9): the FP64 view as one 15-wide segment instead of two 8-node
+    // segments with x1 halos (p64's bands are padded to 16 rows, the pencil reads 15)
+    PencilView<double> p64w{};
+    bool p64w_ok = false;
     int G = kGroups;  // workgroups per tile of the node-parallel kernels (set at creation)
     int nfold = 0;    // > 0: block totals of the two-level partial sum (G > kFoldAbove)
     int fchunk = 0;   // groups per first-level block
@@ -443,6 +447,10 @@ struct Engine {
                         hipLaunchKernelGGL((k_pencil<double, T, 1, true>), grid, dim3(kBlock),
                                            pencil_lds_bytes(op.p64), st, op.p64, op.NN, x64, (double*)nullptr, b64, r,
                                            cs, rel, part, dinv64);
+                    else if (op.p64w_ok)  // one 15-wide segment, no x1 halos
+                        hipLaunchKernelGGL((k_pencil<double, T, 1, true, 15>), grid, dim3(kBlock),
+                                           (pencil_lds_bytes<double, 15>(op.p64w)), st, op.p64w, op.NN, x64,
+                                           (double*)nullptr, b64, r, cs, rel, part, dinv64);
                     else
                         hipLaunchKernelGGL((k_pencil<double, T, 1, false>), grid, dim3(kBlock),
                                            pencil_lds_bytes(op.p64), st, op.p64, op.NN, x64, (double*)nullptr, b64, r,
@@ -1213,6 +1221,11 @@ eqd_status eqd_operator_create(const eqd_grid_desc* g, eqd_op** out) {
         op->v32 = op->view<float>();
         op->p64 = op->pview<double>();
         op->p32 = op->pview<float>();
+        if (op->tb.NX > pencil_s<double>() && op->tb.NX <= 15) {
+            op->p64w = op->p64;
+            op->p64w.nseg = 1;
+            op->p64w_ok = true;
+        }
         // Groups per tile: about 4 FP32 pencils per workgroup (one per wave), at least 64 and a
         // multiple of 64. With one pencil per wave the blocks of a tile sweep its pencils in
         // order and are short, so the pencils in flight on an XCD are one window of neighbours

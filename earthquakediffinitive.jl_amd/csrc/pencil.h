@@ -315,10 +315,18 @@ __device__ __forceinline__ void pencil_apply(const PencilView<T>& op, const XBan
 // MODE 1: reliable-update residual r = b - A x (FP64 in, r stored as R for flagged
 //         lanes) with the weighted partial r.r of the FP64 residual (r.(dinv r) when a
 //         Jacobi dinv is given).
+// Waves per SIMD a pencil instantiation is built for: full-line FP32 pencils keep 4, pencils
+// with x1 halos 3, and an FP64 pencil wider than 8 nodes 2 (its accumulators alone take 90 VGPRs).
+template <typename T, bool F, int SW>
+constexpr int pencil_waves() {
+    return (sizeof(T) == 8 && SW > 8) ? 2 : (F ? EQD_PENCIL_WAVES : EQD_PENCIL_WAVES - 1);
+}
+
 // SW: pencil width (default: pencil_s<T>()); a full-line pencil of an x1 line shorter than that
-// can be instantiated narrower so that no load falls past the line's end.
+// can be instantiated narrower so that no load falls past the line's end, and an FP64 full-line
+// pencil wider than pencil_s<double>() replaces two 8-node segments with halos.
 template <typename T, typename R, int MODE, bool F, int SW = pencil_s<T>()>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(F ? EQD_PENCIL_WAVES : EQD_PENCIL_WAVES - 1))) void k_pencil(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(pencil_waves<T, F, SW>()))) void k_pencil(
     PencilView<T> op, int NN, const T* __restrict__ in, T* __restrict__ q, const T* __restrict__ b,
     R* __restrict__ rout, const ColState* __restrict__ cs, const int* __restrict__ tile_on,
     double* __restrict__ partial, const double* __restrict__ dinv) {
@@ -394,10 +402,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(F ? EQD_
     block_partial<64>(acc, partial, tile, g);
 }
 
-// Dynamic LDS bytes k_pencil needs for the x1 bands of a view.
-template <typename T>
+// Dynamic LDS bytes k_pencil needs for the x1 bands of a view (pencil width S).
+template <typename T, int S = pencil_s<T>()>
 inline size_t pencil_lds_bytes(const PencilView<T>& op) {
-    const size_t rows = (size_t)op.nseg * pencil_s<T>();
+    const size_t rows = (size_t)op.nseg * S;
     return rows * ((2 * kRxS + 1) + 2 * (2 * kRxO + 1)) * sizeof(T);
 }
 
