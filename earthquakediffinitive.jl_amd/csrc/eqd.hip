@@ -374,14 +374,23 @@ struct Engine {
         };
         timed(0, [&] {
             if constexpr (K == 64) {
-                PencilView<T> pv;
-                if constexpr (std::is_same<T, float>::value) pv = op.p32; else pv = op.p64;
+                PencilView<T> pv, pvw{};
+                bool wide = false;
+                if constexpr (std::is_same<T, float>::value) pv = op.p32; else {
+                    pv = op.p64;
+                    pvw = op.p64w;
+                    wide = op.p64w_ok;
+                }
                 // x1 lines of at most 15 nodes (n1 = 9: the c1, c2 grids) take a 15-wide pencil, so
                 // that no neighbour-line load falls past the line's end (stencil 559 -> 535 us)
                 if (pv.nseg == 1 && std::is_same<T, float>::value && pv.NX <= 15)
                     hipLaunchKernelGGL((k_pencil<T, T, 0, true, 15>), grid, dim3(kBlock), pencil_lds_bytes(pv), st,
                                        pv, op.NN, p, q, (const T*)nullptr, (T*)nullptr, cs, on, part,
                                        (const double*)nullptr);
+                else if (wide)  // FP64 on 15-node lines: one segment instead of two with x1 halos
+                    hipLaunchKernelGGL((k_pencil<T, T, 0, true, 15>), grid, dim3(kBlock),
+                                       (pencil_lds_bytes<T, 15>(pvw)), st, pvw, op.NN, p, q, (const T*)nullptr,
+                                       (T*)nullptr, cs, on, part, (const double*)nullptr);
                 else if (pv.nseg == 1)
                     hipLaunchKernelGGL((k_pencil<T, T, 0, true>), grid, dim3(kBlock), pencil_lds_bytes(pv), st, pv,
                                        op.NN, p, q, (const T*)nullptr, (T*)nullptr, cs, on, part,
