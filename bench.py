@@ -61,16 +61,22 @@ def build_device(solver, Kbuf, nf2):
 
 
 def copy_calibration(mib):
-    """A plain copy kernel of known size (reads and writes mib MiB per launch)."""
-    import torch
-    n = mib * 2**20 // 4
-    a = torch.ones(n, device="cuda", dtype=torch.float32)
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
+    """A plain device-to-device copy of known size (mib MiB read and written per launch) for the
+    counter calibration: hipMemcpy between two device buffers runs as one copy kernel on the
+    device (the runtime's buffer-copy blit), driven here through the HIP runtime directly."""
+    n = mib * 2**20
+    src, dst = DeviceBuffer(n), DeviceBuffer(n)
+    hip = src.hip
+    if hip.hipMemset(src.ptr, ctypes.c_int(1), ctypes.c_size_t(n)) != 0:
+        raise RuntimeError("hipMemset failed")
     for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_mib": mib, "bytes_read_per_launch": n * 4, "bytes_written_per_launch": n * 4}))
+        if hip.hipMemcpy(dst.ptr, src.ptr, ctypes.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    if hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
+    src.free()
+    dst.free()
+    print(json.dumps({"copy_calibration_mib": mib, "bytes_read_per_launch": n, "bytes_written_per_launch": n}))
 
 
 def measured_traffic(kernel):
@@ -340,12 +346,15 @@ def bench_ensemble(args):
         print(f"bench: ensemble PW {args.ens_members} members, {args.ens_pw_hours} h ...", file=sys.stderr,
               flush=True)
         t0 = time.perf_counter()
-        stp = epw.run(tspan=(0.0, args.ens_pw_hours * 3600.0), saveat=600.0)
+        # a step budget per member: PW steps grow steeply with the horizon, so a long horizon ends
+        # with a MaxIters retcode (counted below) instead of an unbounded run
+        stp = epw.run(tspan=(0.0, args.ens_pw_hours * 3600.0), saveat=600.0, maxiters=args.ens_pw_maxiters)
         t_pw = time.perf_counter() - t0
         accp = [s.naccept for s in stp]
         pw = {"hours": args.ens_pw_hours, "run_ms": round(t_pw * 1e3, 1), "device_loop_ms": round(epw.loop_ms, 1),
               "accepted_steps_min_max": [int(min(accp)), int(max(accp))],
               "member_steps_per_s": round(sum(accp) / (epw.loop_ms * 1e-3), 1),
+              "maxiters": args.ens_pw_maxiters, "members_at_maxiters": int(sum(s.retcode != 0 for s in stp)),
               "retcodes_ok": bool(all(s.retcode == 0 for s in stp))}
         print(f"bench: ensemble PW {args.ens_members} members {args.ens_pw_hours} h: loop {epw.loop_ms:.0f} ms, "
               f"steps {min(accp)}-{max(accp)}", file=sys.stderr, flush=True)
@@ -471,6 +480,8 @@ def main():
     ap.add_argument("--ens_hours", type=float, default=720.0, help="ensemble horizon (30 days)")
     ap.add_argument("--ens_pw_hours", type=float, default=3.0,
                     help="horizon of the Peaceman-well ensemble (0: skip; its steps grow fast with the horizon)")
+    ap.add_argument("--ens_pw_maxiters", type=int, default=3000,
+                    help="step budget per Peaceman-well member (a longer run returns MaxIters)")
     ap.add_argument("--serve_seconds", type=float, default=10.0)
     ap.add_argument("--serve_rate", type=float, default=10.0,
                     help="mean request rate; 50 req/s over this grid mix exceeds one GPU (see DESIGN.md)")

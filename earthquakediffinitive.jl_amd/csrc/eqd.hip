@@ -36,6 +36,14 @@ void validate_grid(const eqd_grid_desc& g) {
         throw Error(EQD_ERR_GRID, "SBP order 4 needs at least 9 points per dimension (n1 = " + std::to_string(g.n1) +
                                       ", n23 = " + std::to_string(g.n23) + ")");
     if (!(g.L_normal > 0) || !(g.L_fault > 0) || !(g.mu > 0)) throw Error(EQD_ERR_ARG, "L_normal, L_fault, mu must be > 0");
+    // The tile kernels address a 64-column tile vector through a buffer resource with 32-bit
+    // byte offsets: the largest one (FP64, 3 components per stored node) must stay below 2^31 B.
+    const int64_t NN = (int64_t)(2 * g.n1 - 3) * (g.n23 - 2) * (g.n23 - 2);
+    const int64_t tile_bytes = NN * 3 * 64 * (int64_t)sizeof(double);
+    if (tile_bytes > (int64_t)INT32_MAX)
+        throw Error(EQD_ERR_GRID, "grid too large: " + std::to_string(NN) + " stored nodes make a 64-column FP64 tile of " +
+                                      std::to_string(tile_bytes) + " B, above the 2^31 B a tile vector may span (at most " +
+                                      std::to_string((int64_t)INT32_MAX / (3 * 64 * 8)) + " stored nodes)");
 }
 
 // Omega_f from l_f as src/FaultResponse.jl:114-126 finds it.
@@ -1160,7 +1168,7 @@ int64_t serve_grid_step(eqd_server* sv, ServeGrid& G) {
 
 extern "C" {
 
-const char* eqd_version(void) { return "eqd 0.1 (gfx950)"; }
+const char* eqd_version(void) { return "libeqd 0.2.0 (gfx950)"; }
 
 int64_t eqd_tables_export(const eqd_grid_desc* g, int which, int* start, int* width, double* coef) {
     int64_t n = -1;

@@ -61,13 +61,18 @@ def _domain_line(m) -> str:
             f"SBP order {gi['order']}, {gi['elastic_dofs']} elastic DOF")
 
 
+# The code that writes these files (the format's `code=` / `version=` header keys).
+CODE_NAME = "libeqd"
+CODE_VERSION = "0.2.0"
+
+
 def _date() -> str:
     return datetime.date.today().strftime("%Y/%m/%d")
 
 
 def _ts_header(m, modeler, location, extra):
     return (["# This is the file header:", f"# problem=SEAS Benchmark {_problem_name(m)}",
-             "# code=EarthquakeDiffinitive", "# version=0.1.0", f"# modeler={modeler}", f"# date={_date()}",
+             f"# code={CODE_NAME}", f"# version={CODE_VERSION}", f"# modeler={modeler}", f"# date={_date()}",
              f"# element_size={_fmt_real(m.grid_info['dz'])} m", f"# location={location}"]
             + list(extra) + [_domain_line(m)])
 
@@ -198,8 +203,8 @@ def _write_profiles(m, sol, dir, profile_dt, strike, depth, line_fields, glob, m
                 io.write(f"# problem=SEAS Benchmark {_problem_name(m)}\n")
                 io.write(f"# author={modeler}\n")
                 io.write(f"# date={_date()}\n")
-                io.write("# code=EarthquakeDiffinitive\n")
-                io.write("# code_version=0.1.0\n")
+                io.write(f"# code={CODE_NAME}\n")
+                io.write(f"# code_version={CODE_VERSION}\n")
                 io.write(f"# element_size={_fmt_real(m.grid_info['dz'])} m\n")
                 io.write(f"# Row #1 = {'Strike' if lname == 'strike' else 'Depth'} (m) with two zeros first\n")
                 io.write("# Column #1 = Time (s)\n")

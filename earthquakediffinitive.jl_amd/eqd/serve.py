@@ -8,6 +8,7 @@ whether it runs alone or among others, and the same as the batch build's columns
 from __future__ import annotations
 
 import ctypes as C
+import time
 
 import numpy as np
 
@@ -88,12 +89,16 @@ class Server:
         check(lib().eqd_server_result(self.handle, rid, out.ctypes.data, n2, iters.ctypes.data, C.byref(unc)))
         return out, iters, unc.value
 
-    def drain(self, max_steps: int = 100000):
-        """Steps until nothing is queued or in flight; returns the completed ids."""
+    def drain(self, timeout_s: float = 600.0):
+        """Steps until nothing is queued or in flight; returns the completed ids. A step returns at
+        once for a grid whose previous replay is still running, so the bound is wall-clock time,
+        not a count of steps."""
         done = []
-        for _ in range(max_steps):
+        t_end = time.monotonic() + timeout_s
+        while True:
             busy, queued = self.step()
             done += self.completed()
             if busy == 0 and queued == 0:
                 return done
-        raise EqdError(_lib.EQD_ERR_DEVICE, "serving loop did not drain")
+            if time.monotonic() > t_end:
+                raise EqdError(_lib.EQD_ERR_DEVICE, f"serving loop did not drain within {timeout_s} s")

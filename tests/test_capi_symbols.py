@@ -47,3 +47,18 @@ def test_operator_create_rejects_bad_patch_before_touching_device():
     with pytest.raises(eqd.EqdError) as e:
         eqd.bp8_grid_pair(100.0, 800.0, 400.0)   # BASELINE c1 as written: n1 = 5 < 9
     assert e.value.code == _lib.EQD_ERR_GRID
+
+
+def test_operator_create_rejects_grid_beyond_tile_offsets():
+    """A 64-column FP64 tile vector is addressed with 32-bit byte offsets: a grid whose tile would
+    span 2^31 B or more is refused with EQD_ERR_GRID before any device call (dz = 20 m on the
+    L = (1600, 800) domain: 1,997,199 stored nodes, 3.07 GB per tile); dz = 25 m there (1,016,127
+    stored nodes, 1.56 GB) is below the limit and gets as far as the device check."""
+    import eqd
+    with pytest.raises(eqd.EqdError) as e:
+        eqd.FaultElasticity(eqd.bp8_grid_pair(20.0, 1600.0, 800.0), 32.04e9, 32.04e9, l_f=1600.0)
+    assert e.value.code == _lib.EQD_ERR_GRID and "grid too large" in str(e.value)
+    try:
+        eqd.FaultElasticity(eqd.bp8_grid_pair(25.0, 1600.0, 800.0), 32.04e9, 32.04e9, l_f=1600.0)
+    except eqd.EqdError as e2:
+        assert e2.code == _lib.EQD_ERR_DEVICE  # no GPU in this process

@@ -2,8 +2,10 @@
 
 Bars: rhs!/evaluate! entries within 1e-10 relative (the Newton stops at |d ln V| < 1e-12;
 the K product and A_p differ from the oracle only in summation order); Tsit5 runs within
-1e-6 relative in slip and pressure and 1e-5 in V_max at every saved time (the step
-sequences agree, the states differ by rounding amplified by the adaptive controller);
+1e-6 relative in slip and pressure, and in V_max at every saved time within 1e-6 or, where the
+trajectory itself is that ill-conditioned, twice the oracle's own spread under ulp-level
+scalings of K (the step sequences agree, the states differ by rounding amplified by the
+adaptive controller);
 the published 100 h scalars (PROGRESS.md:584-593) end to end with K built on the device.
 """
 import math
@@ -102,9 +104,27 @@ def _run_pair(K, tf, saveat, injection="gaussian"):
     return m, sol, ro
 
 
+SCALINGS = (1 + 1e-15, 1 - 1e-15, 1 + 4e-15, 1 - 4e-15)
+
+
+def vmax_envelope(K, ro, tf, saveat):
+    """Per save, the bound a device V_max(t) is held to: max(1e-6, 2 x the oracle's own spread
+    when K is scaled by 1 +- 1e-15 and 1 +- 4e-15 -- a few ulps, the size of any change of
+    summation order). Where the trajectory is well conditioned the spread is ~1e-9 and the bound
+    is 1e-6; only near the V_max minimum after the injection stops does it rise above."""
+    n2f = int(round(math.sqrt(K.shape[0] // 2)))
+    spread = np.zeros_like(ro["vmax"])
+    for f in SCALINGS:
+        r = OracleBP8(K * f, n2f, dict(dz=50.0)).run(0.0, tf, saveat)
+        assert np.array_equal(r["t"], ro["t"])
+        spread = np.maximum(spread, np.abs(r["vmax"] - ro["vmax"]) / ro["vmax"])
+    return np.maximum(1e-6, 2 * spread), spread
+
+
 def test_run_100h_matches_oracle(toeplitz_K):
     print("[bp8 run 100 h toeplitz]", flush=True)
-    m, sol, ro = _run_pair(toeplitz_K, 100 * 3600.0, 3600.0)
+    tf = 100 * 3600.0
+    m, sol, ro = _run_pair(toeplitz_K, tf, 3600.0)
     nf = m.nf
     assert sol.retcode == "Success"
     assert np.array_equal(sol.t, ro["t"])
@@ -113,26 +133,42 @@ def test_run_100h_matches_oracle(toeplitz_K):
     assert close(sol.u[:, 2 * nf:3 * nf], ro["u"][:, 2 * nf:3 * nf], 1e-8)
     assert close(sol.u[:, 3 * nf:], ro["u"][:, 3 * nf:], 1e-6, 1e-3)
     vmax = bp8.saved_vmax(m, len(sol.t))
-    assert close(vmax, ro["vmax"], 1e-5)
+    bound, spread = vmax_envelope(toeplitz_K, ro, tf, 3600.0)
+    d = np.abs(vmax - ro["vmax"]) / ro["vmax"]
+    print(f"  V_max: max rel diff {d.max():.2e}, oracle spread max {spread.max():.2e}, saves held to 1e-6: "
+          f"{int((bound == 1e-6).sum())} of {len(bound)}", flush=True)
+    assert np.all(d <= bound)
+    assert np.all(bound == 1e-6)  # over the first 100 h the north-star bar applies at every save
 
 
 def test_run_30day_published_gs(toeplitz_K):
     """PROGRESS.md:671 (30 days, saveat 300, :toeplitz): peak V 1.10e-6 at 2.22 days, final
-    slip 0.0401 m, peak p(0,0) 13.17 MPa -- and the oracle's trajectory: V_max(t) within 1e-5 at
-    every save (measured 1.3-2e-6; near day 19 the oracle itself moves by 7e-7 when K is scaled by
-    1 + 1e-15, see tests/test_oracle_bp8.py::test_30day_vmax_conditioning)."""
+    slip 0.0401 m, peak p(0,0) 13.17 MPa -- and the oracle's trajectory: V_max(t) at every save
+    within max(1e-6, 2 x the oracle's own spread under K x (1 +- 1e-15), (1 +- 4e-15)). Near day 19
+    (after the injection stops, V ~ 2e-8 m/s) a last-bit change of K moves the oracle itself by
+    ~1e-6 (tests/test_oracle_bp8.py::test_30day_vmax_conditioning); everywhere else the 1e-6 bar
+    holds: the oracle's spread leaves 8337 of the 8641 saves at 1e-6 (all but days 18.5-25.9)."""
     print("[bp8 run 30 d toeplitz]", flush=True)
-    m, sol, ro = _run_pair(toeplitz_K, 30 * 86400.0, 300.0)
+    tf = 30 * 86400.0
+    m, sol, ro = _run_pair(toeplitz_K, tf, 300.0)
     nf = m.nf
     ic = (m.n2f // 2) * (m.n2f + 1)
-    assert len(sol.t) == 8641 and sol.t[-1] == 30 * 86400.0
+    assert len(sol.t) == 8641 and sol.t[-1] == tf
     vmax = bp8.saved_vmax(m, len(sol.t))
     j = int(np.argmax(vmax))
     assert f"{vmax[j]:.2e}" == "1.10e-06" and f"{sol.t[j] / 86400:.2f}" == "2.22"
     assert f"{sol.u[-1, ic]:.4f}" == "0.0401"
     assert f"{sol.u[:, 3 * nf + ic].max() / 1e6:.2f}" == "13.17"
-    assert close(vmax, ro["vmax"], 1e-5)
-    assert close(sol.u[:, :2 * nf], ro["u"][:, :2 * nf], 1e-5, 1e-12)
+    bound, spread = vmax_envelope(toeplitz_K, ro, tf, 300.0)
+    d = np.abs(vmax - ro["vmax"]) / ro["vmax"]
+    n6 = int((bound == 1e-6).sum())
+    k = int(np.argmax(d / bound))
+    print(f"  V_max: max rel diff {d.max():.2e} at day {sol.t[int(np.argmax(d))] / 86400:.2f}; worst ratio to bound "
+          f"{(d / bound)[k]:.2f} at day {sol.t[k] / 86400:.2f}; saves held to 1e-6: {n6} of {len(bound)}; "
+          f"oracle spread max {spread.max():.2e}", flush=True)
+    assert np.all(d <= bound)
+    assert n6 >= 8300
+    assert close(sol.u[:, :2 * nf], ro["u"][:, :2 * nf], 1e-6, 1e-12)
 
 
 def test_build_model_exact_on_device_reproduces_published_100h():

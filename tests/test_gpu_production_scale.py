@@ -6,7 +6,9 @@ properties that do not need the CPU oracle at this size:
 * the Toeplitz build's measured columns (the centre source, both directions) equal the exact build's
   columns to 1e-6 (test/fault_response_test.jl:128-171 pins the same on its small fixture);
 * negative self-stiffness at every sampled node and zero slip -> exactly zero traction;
-* batching invariance at this grid: a column built alone equals the same column in a batch, bitwise.
+* batching invariance at this grid: a column built alone equals the same column in a batch, bitwise;
+* four columns of the seeded exact subset within 1e-8 of the CPU oracle (a committed fixture: the
+  oracle needs ~80 s per column at this size).
 """
 import numpy as np
 import pytest
@@ -58,3 +60,20 @@ def test_toeplitz_measured_columns_are_exact(grid):
     assert np.all(np.diag(KT) < 0)
     z2, z3 = eqd.shear_traction(fe2, np.zeros(nf), np.zeros(nf))
     assert not z2.any() and not z3.any()
+
+
+@pytest.mark.parametrize("prec,tile_k", [("mixed", 64), ("fp64", 8)])
+def test_exact_subset_against_oracle_fixture(grid, prec, tile_k):
+    """Four columns of the production benchmark's seeded 512-column exact subset against the CPU
+    oracle's FP64 columns (tests/golden/c3_columns.npz, written by make_c3_columns.py): within
+    1e-8 relative, per column."""
+    import os
+    print(f"[production dz=25: 4 columns {prec} vs oracle fixture]", flush=True)
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c3_columns.npz"))
+    cols, Kref = z["cols"], z["K"]
+    fe = eqd.FaultElasticity(grid, LAM, MU, l_f=1600.0, precision=prec, tile_k=tile_k)
+    K = eqd.fault_stiffness(fe, cols=cols)
+    assert K.shape == Kref.shape == (33282, 4)
+    err = np.linalg.norm(K - Kref, axis=0) / np.linalg.norm(Kref, axis=0)
+    print(f"  per-column rel err {err}", flush=True)
+    assert err.max() <= 1e-8
