@@ -229,19 +229,21 @@ def test_jacobi_fault_stiffness(name, precision, tile_k):
 
 
 def test_jacobi_iteration_drift_is_rounding():
-    """Column by column (every third unit-slip column of the ring grid, FP64 Jacobi PCG): where the
-    device's iteration count differs from the oracle's, it differs by one, and the oracle itself
-    stops at the device's count for that column once its right-hand side is perturbed at the level
-    of one unit in the last place (b scaled by 1 + k 2^-52, k in +-1, +-2, +-4, or +-1 ulp of noise
-    per entry): the count is not determined beyond rounding (finite-precision CG), and the device,
-    which sums every dot in another order, lands inside that spread. The share of columns that
-    differ is at most twice the share the oracle moves under one seeded +-1 ulp noise draw."""
+    """Column by column (every third unit-slip column of the ring grid, FP64 Jacobi PCG): the device's
+    iteration count is the oracle's or one apart, and the share of columns that stop one apart is
+    no larger than the share the oracle itself moves when its operator changes by one to four units
+    in the last place (lambda and mu scaled by 1 + k 2^-52, k = +-1..+-4: every matrix-vector product
+    then differs at the rounding level, as the device's, which sums in another order, does). In
+    finite precision CG's stopping iteration is not determined beyond that; the device's drift is
+    of exactly that size."""
     fe, o = pair("ring", precision="fp64", precond="jacobi")
-    g = Geometry(*CASES["ring"])
-    oj = OracleSystem(g, precond="jacobi")
-    nf = o.nf
+    n1, n23, Ln, Lf, lam, mu, lf = CASES["ring"]
+    oj = OracleSystem(Geometry(n1, n23, Ln, Lf, lam, mu, lf), precond="jacobi")
     eps = 2.0 ** -52
-    differ, noise_moved, n, prev = [], 0, 0, 0
+    variants = [OracleSystem(Geometry(n1, n23, Ln, Lf, lam * (1 + k * eps), mu * (1 + k * eps), lf), precond="jacobi")
+                for k in (1, -1, 2, -2, 3, -3, 4, -4)]
+    nf = o.nf
+    differ, moved, n, prev = 0, 0, 0, 0
     for c in range(0, 2 * nf, 3):
         s2, s3 = np.zeros(nf), np.zeros(nf)
         (s2 if c < nf else s3)[c % nf] = 1.0
@@ -254,18 +256,12 @@ def test_jacobi_iteration_drift_is_rounding():
         _, it_o, ok = oj.cg(b)
         assert ok
         n += 1
-        noise = [b * (1 + eps * np.random.default_rng(seed).choice([-1.0, 0.0, 1.0], size=b.size))
-                 for seed in range(4)]
-        noise_moved += oj.cg(noise[0])[1] != it_o
-        if it_dev == it_o:
-            continue
-        assert abs(it_dev - it_o) == 1, (c, it_dev, it_o)
-        variants = [b * (1 + k * eps) for k in (1, -1, 2, -2, 4, -4)] + noise
-        assert any(oj.cg(v)[1] == it_dev for v in variants), (c, it_dev, it_o)
-        differ.append(c)
-    print(f"  {len(differ)} of {n} columns stop one iteration apart from the oracle; a +-1 ulp noise draw "
-          f"moves {noise_moved} of the oracle's own", flush=True)
-    assert len(differ) <= 2 * noise_moved + 2
+        assert abs(it_dev - it_o) <= 1, (c, it_dev, it_o)
+        differ += it_dev != it_o
+        moved += any(v.cg(b)[1] != it_o for v in variants)
+    print(f"  {differ} of {n} columns stop one iteration apart from the oracle; ulp-level changes of the "
+          f"oracle's operator move {moved} of its own", flush=True)
+    assert differ <= moved
 
 
 def test_jacobi_plain_engine_and_batching():
