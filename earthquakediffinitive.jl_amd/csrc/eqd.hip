@@ -630,6 +630,36 @@ struct Engine {
         s->last_tile_iters = tile_iters;
     }
 
+    // The same iterations as solve(), launched one by one instead of through a captured graph
+    // and without events or pinned host memory: tile activity is polled into a host word every
+    // `gi` iterations (the allocation-free entry point's loop).
+    static void solve_direct(eqd_solver* s) {
+        Workspace& w = *s->ws;
+        hipStream_t st = s->stream;
+        const int gi = s->o.graph_iters > 0 ? s->o.graph_iters : 8;
+        const int64_t itmax = s->o.itmax > 0 ? s->o.itmax : 2 * s->op->ntot;
+        int64_t chunks = 0;
+        for (;;) {
+            std::vector<int> on(w.ntiles, 0);
+            HIPCHK(hipMemcpyAsync(on.data(), w.tile_on.p, w.ntiles * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            int act = 0;
+            for (int v : on) act += v != 0;
+            if (!act) break;
+            if (chunks * gi > itmax + gi) throw Error(EQD_ERR_DEVICE, "CG iteration guard exceeded");
+            for (int i = 0; i < gi; ++i) iteration(s, st);
+            ++chunks;
+        }
+        if (std::is_same<T, float>::value) {  // fold any correction still held in FP32
+            dim3 grid(s->op->G, w.ntiles);
+            hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, s->op->NN, w.x.as<double>(),
+                               w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1, (const int*)nullptr);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        s->last_launches = chunks * gi;
+    }
+
     // Plain engine: one column, host-side alpha/beta (device->host read of the partial
     // sums every half iteration), no graphs -- the one-RHS baseline.
     static void solve_plain(eqd_solver* s) {
@@ -1511,6 +1541,150 @@ eqd_status eqd_shear_traction(eqd_solver* s, const double* s2, const double* s3,
         s->o = saved;
         std::vector<double> h((size_t)2 * op.nf);
         HIPCHK(hipMemcpy(h.data(), out.p, h.size() * 8, hipMemcpyDeviceToHost));
+        std::memcpy(dtau2, h.data(), op.nf * 8);
+        std::memcpy(dtau3, h.data() + op.nf, op.nf * 8);
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// Allocation-free forward solve. The Krylov workspace of a one-column solve lives in a static
+// device array (one copy per device, held until the process ends) instead of the solver's
+// hipMalloc'd workspace; the iterations are launched directly (no graph, no events, no pinned
+// host memory). Calls that overlap on one device share the array and race.
+
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+// The array's address on each device: the host-side name of a __device__ variable is a
+// stand-in, so every kernel argument and copy uses the address looked up here.
+void* scratch_ptr() {
+    static std::mutex mu;
+    static std::map<int, void*> addr;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(dev);
+    if (it != addr.end()) return it->second;
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(EQD_ERR_DEVICE, "hipGetSymbolAddress returned a null address for the scratch array");
+    addr[dev] = p;
+    return p;
+}
+
+__global__ void k_zero_words(uint32_t* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Carves 256-byte-aligned pieces off the scratch array, in order (dry: only measures).
+struct Carver {
+    unsigned char* base;
+    bool dry;
+    size_t off = 0;
+    void take(DevMem& m, size_t n) {
+        off = (off + 255) & ~(size_t)255;
+        if (!dry) m.borrow(base + off, n);
+        off += n;
+    }
+    size_t aligned() const { return (off + 255) & ~(size_t)255; }
+};
+
+// Lays a one-column (K = 1) workspace of the solver's precision out in the scratch array (with
+// base == nullptr it only measures) and returns the bytes it spans. `out` receives the traction
+// column (2nf doubles); `ctl` spans the control words (partial sums, column state, tile flags,
+// ids), which are cleared on every call.
+size_t carve_noalloc(const eqd_op& op, int prec, unsigned char* base, Workspace& w, DevMem& out, DevMem& ctl) {
+    const size_t nvec = (size_t)op.NN * 3;
+    const size_t t = prec == EQD_PREC_MIXED ? sizeof(float) : sizeof(double);
+    Carver c{base, base == nullptr};
+    c.take(w.b, nvec * 8);
+    c.take(w.x, nvec * 8);
+    c.take(w.r, nvec * t);
+    c.take(w.p, nvec * t);
+    c.take(w.q, nvec * t);
+    if (prec == EQD_PREC_MIXED) c.take(w.xc, nvec * sizeof(float));
+    c.take(w.fields, 2 * (size_t)op.g.n23 * op.g.n23 * 8);
+    c.take(out, 2 * (size_t)op.nf * 8);
+    const size_t lo = c.aligned();
+    c.take(w.partial, (size_t)op.G * 8);
+    if (op.nfold) c.take(w.folded, (size_t)op.nfold * 8);
+    c.take(w.cs, sizeof(ColState));
+    c.take(w.tile_on, sizeof(int));
+    c.take(w.tile_rel, sizeof(int));
+    c.take(w.colid, 8);
+    c.take(w.outpos, 8);
+    const size_t hi = c.aligned();
+    if (!c.dry) ctl.borrow(base + lo, hi - lo);
+    return hi;
+}
+
+}  // namespace
+
+extern "C" {
+
+eqd_status eqd_shear_traction_noalloc(eqd_solver* s, const double* s2, const double* s3, double* dtau2,
+                                      double* dtau3) {
+    return guard([&] {
+        if (!s || !s2 || !s3 || !dtau2 || !dtau3) throw Error(EQD_ERR_ARG, "null argument");
+        if (s->o.engine != EQD_ENGINE_BATCHED)
+            throw Error(EQD_ERR_ARG, "the allocation-free forward solve runs the batched engine (engine = BATCHED)");
+        const eqd_op& op = *s->op;
+        const int prec = s->o.precision;
+        std::unique_ptr<Workspace> own(new Workspace());
+        Workspace& w = *own;
+        DevMem out, ctl;
+        const size_t need = carve_noalloc(op, prec, nullptr, w, out, ctl);
+        if (need > kScratchBytes)
+            throw Error(EQD_ERR_RANGE, "the allocation-free forward solve needs " + std::to_string(need) +
+                                           " B of device scratch for this grid; its static scratch holds " +
+                                           std::to_string(kScratchBytes) + " B (64 MiB, about 1.6M stored unknowns "
+                                           "in FP64); use eqd_shear_traction");
+        carve_noalloc(op, prec, static_cast<unsigned char*>(scratch_ptr()), w, out, ctl);
+        w.K = 1;
+        w.ntiles = 1;
+        w.prec = prec;
+        w.nvec = (size_t)op.NN * 3;
+        hipStream_t st = s->stream;
+        const int n23 = op.g.n23;
+        std::vector<double> f(2 * (size_t)n23 * n23);
+        slip_fields(op, s2, s3, f.data());
+        hipLaunchKernelGGL(k_zero_words, dim3(64), dim3(256), 0, st, ctl.as<uint32_t>(), ctl.bytes / 4);
+        HIPCHK(hipGetLastError());
+        const int64_t cid = 0, opos = 0;
+        HIPCHK(hipMemcpyAsync(w.colid.p, &cid, 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(w.outpos.p, &opos, 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(w.fields.p, f.data(), f.size() * 8, hipMemcpyHostToDevice, st));
+        // the solver's own workspace is set aside for the call and put back afterwards
+        own.swap(s->ws);
+        struct Restore {
+            eqd_solver* s;
+            std::unique_ptr<Workspace>& own;
+            ~Restore() { own.swap(s->ws); }
+        } restore{s, own};
+        with_k(1, [&](auto d) {
+            decltype(d)::rhs(s);
+            decltype(d)::init(s);
+            if (prec == EQD_PREC_MIXED) Engine<float, 1>::solve_direct(s);
+            else Engine<double, 1>::solve_direct(s);
+            decltype(d)::extract(s, out.as<double>(), 2 * op.nf);
+        });
+        ColState c;
+        HIPCHK(hipMemcpyAsync(&c, s->ws->cs.p, sizeof(c), hipMemcpyDeviceToHost, st));
+        std::vector<double> h((size_t)2 * op.nf);
+        HIPCHK(hipMemcpyAsync(h.data(), out.p, h.size() * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        s->st.solves += 1;
+        s->st.iterations += c.iters;
+        s->st.max_iterations = std::max<int64_t>(s->st.max_iterations, c.iters);
+        if (c.status != kSolved) {
+            s->st.unconverged += 1;
+            if (s->st.unconverged == 1)
+                std::fprintf(stderr, "warning: CG did not converge on a split-node solve after %d iterations\n", c.iters);
+        }
         std::memcpy(dtau2, h.data(), op.nf * 8);
         std::memcpy(dtau3, h.data() + op.nf, op.nf * 8);
     });

@@ -45,21 +45,31 @@ eqd_status guard(F&& f) {
     }
 }
 
-// Owning device allocation.
+// Device memory: an owning allocation, or (borrow) a view of memory owned elsewhere, which
+// release() leaves alone.
 struct DevMem {
     void* p = nullptr;
     size_t bytes = 0;
+    bool owned = false;
     DevMem() = default;
     explicit DevMem(size_t n) { alloc(n); }
     void alloc(size_t n) {
         release();
         if (n) HIPCHK(hipMalloc(&p, n));
         bytes = n;
+        owned = n != 0;
+    }
+    void borrow(void* q, size_t n) {
+        release();
+        p = q;
+        bytes = n;
+        owned = false;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        owned = false;
     }
     ~DevMem() { release(); }
     DevMem(const DevMem&) = delete;

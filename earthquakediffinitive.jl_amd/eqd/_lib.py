@@ -26,7 +26,7 @@ EXPORTED = [
     "eqd_fault_grid_axes", "eqd_apply_A", "eqd_build_rhs", "eqd_jacobi_preconditioner", "eqd_solver_create",
     "eqd_solver_destroy",
     "eqd_solver_duplicate", "eqd_solver_merge_stats", "eqd_solver_report", "eqd_split_node_solve",
-    "eqd_shear_traction", "eqd_fault_stiffness", "eqd_fault_stiffness_device", "eqd_solver_last_timing", "eqd_solver_kernel_timing",
+    "eqd_shear_traction", "eqd_shear_traction_noalloc", "eqd_fault_stiffness", "eqd_fault_stiffness_device", "eqd_solver_last_timing", "eqd_solver_kernel_timing",
     "eqd_tables_export", "eqd_version", "eqd_last_error",
     "eqd_bp8_default_params", "eqd_bp8_create", "eqd_bp8_create_device", "eqd_bp8_create_from_solver",
     "eqd_bp8_destroy", "eqd_bp8_sizes", "eqd_bp8_stiffness", "eqd_bp8_operators", "eqd_bp8_initial_state",
@@ -112,6 +112,7 @@ def lib():
         L.eqd_solver_report.argtypes = [vp, C.POINTER(Report)]
         L.eqd_split_node_solve.argtypes = [vp, dp, dp]
         L.eqd_shear_traction.argtypes = [vp, dp, dp, dp, dp]
+        L.eqd_shear_traction_noalloc.argtypes = [vp, dp, dp, dp, dp]
         L.eqd_fault_stiffness.argtypes = [vp, C.c_int, vp, C.c_int, vp, C.c_int64]
         L.eqd_fault_stiffness_device.argtypes = [vp, C.c_int, vp, C.c_int, vp, C.c_int64]
         L.eqd_solver_last_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64),

@@ -122,6 +122,18 @@ eqd_status eqd_split_node_solve(eqd_solver* s, const double* rhs, double* u);
 /* shear_traction(fe, s2, s3) -> (dtau2, dtau3) (src/FaultResponse.jl:194-211). */
 eqd_status eqd_shear_traction(eqd_solver* s, const double* s2, const double* s3, double* dtau2, double* dtau3);
 
+/* No counterpart in the reference: eqd_shear_traction for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same one-column solve and the same kernels (results equal to eqd_shear_traction's on a
+ * solver with the same options), but the Krylov workspace lives in a static 64 MiB device array
+ * (one per device, held until the process ends) and the iterations are launched directly: no
+ * hipMalloc, hipFree, graph, event or pinned host memory. Batched engine only. A grid whose
+ * one-column workspace exceeds 64 MiB (about 1.6M stored unknowns in FP64, 2M in mixed
+ * precision) is refused with EQD_ERR_RANGE. Calls that overlap on one device share the array
+ * and race. */
+eqd_status eqd_shear_traction_noalloc(eqd_solver* s, const double* s2, const double* s3, double* dtau2,
+                                      double* dtau3);
+
 enum { EQD_STIFFNESS_EXACT = 0, EQD_STIFFNESS_TOEPLITZ = 1 };
 /* fault_stiffness(fe) / fault_stiffness_toeplitz(fe) (src/FaultResponse.jl:229-287, 339-393).
  * EXACT: cols == NULL builds all 2nf columns into K (ldK >= 2nf, 2nf columns);
