@@ -359,11 +359,38 @@ def bench_ensemble(args):
         print(f"bench: ensemble PW {args.ens_members} members {args.ens_pw_hours} h: loop {epw.loop_ms:.0f} ms, "
               f"steps {min(accp)}-{max(accp)}", file=sys.stderr, flush=True)
         del epw, mpw
+    # BP8-QD-PW over the full 30 days with the implicit-local integrator (per-node 3 x 3 Newton
+    # blocks inside L-stable SDIRK4 stages), one trajectory per member: the first members of the
+    # same seeded set, each a build_model with its own Q0, a and sigma0 (Toeplitz K s)
+    pw_imex = None
+    if args.ens_pw_imex_members > 0:
+        runs = []
+        t0 = time.perf_counter()
+        for j in range(args.ens_pw_imex_members):
+            pj = bp8.benchmark_parameters(l_f=800.0, dz=20.0, Q0=float(mem["Q0"][j]), a=float(mem["a"][j]),
+                                          sigma0=float(mem["sigma0"][j]))
+            mj = bp8.build_model(par=pj, dz=20.0, L_fault=800.0, L_normal=400.0, injection="peaceman",
+                                 stiffness="toeplitz", precision="mixed")
+            sol = bp8.run_bp8(mj, tspan=(0.0, args.ens_pw_imex_hours * 3600.0), saveat=3600.0, alg="imex",
+                              maxiters=400000)
+            runs.append(dict(member=j, accepted_steps=int(sol.naccept), rejected=int(sol.nreject), rhs=int(sol.nrhs),
+                             device_ms=round(sol.ms, 1), retcode=sol.retcode,
+                             p_well_max_MPa=round(float(sol.u[:, -1].max()) / 1e6, 3)))
+            print(f"bench: ensemble PW imex member {j}: {sol.naccept} steps, {sol.ms / 1e3:.2f} s", file=sys.stderr,
+                  flush=True)
+            del mj
+        dev_s = sum(r["device_ms"] for r in runs) / 1e3
+        pw_imex = {"hours": args.ens_pw_imex_hours, "members_run": len(runs), "runs": runs,
+                   "member_steps_per_s": round(sum(r["accepted_steps"] for r in runs) / dev_s, 1),
+                   "member_hours_per_s": round(len(runs) * args.ens_pw_imex_hours / dev_s, 1),
+                   "wall_s": round(time.perf_counter() - t0, 2),
+                   "note": "one trajectory per member (eqd_bp8_integrate_imex); Tsit5 needs ~1e6 steps per 100 h at "
+                           "dz = 20 m on this variant"}
     res = {"nf": m.nf, "members": args.ens_members, "hours": args.ens_hours, "K_build_ms": round(t_build * 1e3, 1),
            "run_ms": round(t_run * 1e3, 1), "device_loop_ms": round(ens_loop_ms, 1),
            "accepted_steps_min_max": [int(min(acc)), int(max(acc))],
            "member_steps_per_s": round(sum(acc) / (ens_loop_ms * 1e-3), 1), "single_trajectory": single,
-           "cpu_member_check": cpu_check, "peaceman": pw,
+           "cpu_member_check": cpu_check, "peaceman": pw, "peaceman_imex": pw_imex,
            "gemm": {"bound": "mfma", "achieved": round(tf / 1e12, 2), "peak": 78.6, "unit": "TFLOP/s",
                     "frac": round(tf / 78.6e12, 4), "ms_per_gemm": round(ms_gemm, 3), "flop_per_gemm": flops}}
     print(f"bench: ensemble dz=20 {args.ens_members} members {args.ens_hours} h: K build {t_build * 1e3:.0f} ms, "
@@ -480,6 +507,10 @@ def main():
     ap.add_argument("--ens_hours", type=float, default=720.0, help="ensemble horizon (30 days)")
     ap.add_argument("--ens_pw_hours", type=float, default=3.0,
                     help="horizon of the Peaceman-well ensemble (0: skip; its steps grow fast with the horizon)")
+    ap.add_argument("--ens_pw_imex_members", type=int, default=4,
+                    help="members of the Peaceman-well ensemble run over --ens_pw_imex_hours with the implicit-local "
+                         "integrator, one trajectory each")
+    ap.add_argument("--ens_pw_imex_hours", type=float, default=720.0)
     ap.add_argument("--ens_pw_maxiters", type=int, default=3000,
                     help="step budget per Peaceman-well member (a longer run returns MaxIters)")
     ap.add_argument("--serve_seconds", type=float, default=10.0)

@@ -1088,6 +1088,181 @@ __global__ void k_ens_vmax(const double* __restrict__ Vm, int nf, double* __rest
     if (threadIdx.x == 0) out[(size_t)m * ldo] = red[0];
 }
 
+
+// ---------------------------------------------------------------------------------
+// Implicit-local integrator for BP8-PW (no counterpart in the reference, which scopes it in
+// PROGRESS.md:872-894 and TODO.md:126-153): the stiff mode of the Peaceman variant is the well
+// node's slip against its own self-stiffness (rate K_ii / (dg/dV)), local to one node, so the
+// stage equations of an L-stable SDIRK method are solved by Newton iterations whose Jacobian is
+// the per-node 3 x 3 block of (s2, s3, ln theta): nf independent 3 x 3 solves per iteration, the
+// dense K coupling between nodes and the pore pressure converging as a fixed point.
+//
+// Method: the 5-stage, order-4 SDIRK of Hairer & Wanner (Solving ODEs II, IV.6), gamma = 1/4,
+// stiffly accurate, with its embedded order-3 solution for the error estimate.
+__constant__ double kSdA[5][5] = {{0.25, 0, 0, 0, 0},
+                                  {0.5, 0.25, 0, 0, 0},
+                                  {17.0 / 50.0, -1.0 / 25.0, 0.25, 0, 0},
+                                  {371.0 / 1360.0, -137.0 / 2720.0, 15.0 / 544.0, 0.25, 0},
+                                  {25.0 / 24.0, -49.0 / 48.0, 125.0 / 16.0, -85.0 / 12.0, 0.25}};
+__constant__ double kSdBh[5] = {59.0 / 48.0, -17.0 / 96.0, 225.0 / 32.0, -85.0 / 12.0, 0.0};
+
+// The node's 2 x 2 block of K (dtau_k / ds_m at the node itself), from the row-major K.
+__global__ void k_imex_kdiag(Dev d, double* __restrict__ Kd) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= d.c.nf) return;
+    const size_t n2 = d.c.n2, nf = d.c.nf;
+    Kd[4 * i + 0] = d.KR[i * n2 + i];
+    Kd[4 * i + 1] = d.KR[i * n2 + nf + i];
+    Kd[4 * i + 2] = d.KR[(nf + i) * n2 + i];
+    Kd[4 * i + 3] = d.KR[(nf + i) * n2 + nf + i];
+}
+
+// Stage start: base = u_n + h sum_{j < stage} A[stage][j] F_j, and the Newton start
+// U = base + h gamma Fg (Fg: the last known stage derivative).
+__global__ void k_imex_base(int ns, int stage, double h, const double* __restrict__ un, const double* __restrict__ F,
+                            const double* __restrict__ Fg, double* __restrict__ base, double* __restrict__ U) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= ns) return;
+    double acc = 0.0;
+    for (int j = 0; j < stage; ++j) acc += kSdA[stage][j] * F[(size_t)j * ns + e];
+    const double b = un[e] + h * acc;
+    base[e] = b;
+    U[e] = b + h * 0.25 * Fg[e];
+}
+
+// One Newton iteration on U = base + hg f(U), f(U) in F (just evaluated, with the slip-rate cache
+// of that evaluation): per node, dU = -(I - hg J)^{-1} G on (s2, s3, ln theta) with J the node's
+// 3 x 3 block, dU = -G on the pressure entries; U += dU and partial[i] = the node's sum of
+// (dU / scale)^2, scale = abstol + reltol |u_n|.
+__global__ void k_imex_newton(Dev d, const double* __restrict__ Kd, const double* __restrict__ F,
+                              const double* __restrict__ base, double hg, const double* __restrict__ un,
+                              double reltol, double* __restrict__ U, double* __restrict__ partial) {
+    const Consts& c = d.c;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= c.nf) return;
+    const int nf = c.nf;
+    const int e0 = i, e1 = nf + i, e2 = 2 * nf + i, e3 = 3 * nf + i;
+    const double G0 = U[e0] - base[e0] - hg * F[e0];
+    const double G1 = U[e1] - base[e1] - hg * F[e1];
+    const double G2 = U[e2] - base[e2] - hg * F[e2];
+    const double G3 = U[e3] - base[e3] - hg * F[e3];
+    const double phi = U[e2];
+    // Jacobian block J[r][q], rows (dV2/dt, dV3/dt, dphi/dt), columns (s2, s3, phi)
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, -exp(-phi)}};
+    const double V = d.Vm[i];
+    const double T2 = d.tau2[i] + c.eta * d.V2[i], T3 = d.tau3[i] + c.eta * d.V3[i];
+    const double T = sqrt(T2 * T2 + T3 * T3);
+    if (d.active[i] && T > 0.0) {
+        const double sig = fmax(c.sigma0 - U[e3], c.sigma_min);
+        const double C = exp((c.fs + c.b * (log(c.Vs / c.Dc) + phi)) / c.a);
+        const double u = V * C / (2 * c.Vs);
+        const double r = sqrt(1 + u * u);
+        const double D = c.eta + sig * c.a * (C / (2 * c.Vs)) / r;  // dg/dV
+        const double dVdT = 1.0 / D, dVdphi = -(sig * c.b * u / r) / D;
+        const double n2 = T2 / T, n3 = T3 / T;
+        const double K22 = Kd[4 * i], K23 = Kd[4 * i + 1], K32 = Kd[4 * i + 2], K33 = Kd[4 * i + 3];
+        const double dT2 = n2 * K22 + n3 * K32, dT3 = n2 * K23 + n3 * K33;  // dT / ds2, dT / ds3
+        J[0][0] = dVdT * dT2 * n2 + V * (K22 - n2 * dT2) / T;
+        J[0][1] = dVdT * dT3 * n2 + V * (K23 - n2 * dT3) / T;
+        J[1][0] = dVdT * dT2 * n3 + V * (K32 - n3 * dT2) / T;
+        J[1][1] = dVdT * dT3 * n3 + V * (K33 - n3 * dT3) / T;
+        J[0][2] = dVdphi * n2;
+        J[1][2] = dVdphi * n3;
+        J[2][0] = -dVdT * dT2 / c.D_RS;
+        J[2][1] = -dVdT * dT3 / c.D_RS;
+        J[2][2] -= dVdphi / c.D_RS;
+    }
+    // (I - hg J) x = -G by Gaussian elimination with partial pivoting (3 x 3)
+    double M[3][4];
+    const double Gv[3] = {G0, G1, G2};
+    for (int r = 0; r < 3; ++r) {
+        for (int q = 0; q < 3; ++q) M[r][q] = (r == q ? 1.0 : 0.0) - hg * J[r][q];
+        M[r][3] = -Gv[r];
+    }
+    for (int col = 0; col < 3; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 3; ++r)
+            if (fabs(M[r][col]) > fabs(M[piv][col])) piv = r;
+        if (piv != col)
+            for (int q = 0; q < 4; ++q) {
+                const double t = M[col][q];
+                M[col][q] = M[piv][q];
+                M[piv][q] = t;
+            }
+        for (int r = col + 1; r < 3; ++r) {
+            const double f = M[r][col] / M[col][col];
+            for (int q = col; q < 4; ++q) M[r][q] -= f * M[col][q];
+        }
+    }
+    double x[3];
+    for (int r = 2; r >= 0; --r) {
+        double acc = M[r][3];
+        for (int q = r + 1; q < 3; ++q) acc -= M[r][q] * x[q];
+        x[r] = acc / M[r][r];
+    }
+    const double dU[4] = {x[0], x[1], x[2], -G3};
+    const int es[4] = {e0, e1, e2, e3};
+    double tot = 0.0;
+    for (int q = 0; q < 4; ++q) {
+        const int e = es[q];
+        U[e] += dU[q];
+        const double sc = abstol_of(c, e) + reltol * fabs(un[e]);
+        const double r = dU[q] / sc;
+        tot += r * r;
+    }
+    if (c.injection == 1 && i == c.well_cell) {
+        const int e = 4 * nf;
+        const double G = U[e] - base[e] - hg * F[e];
+        U[e] -= G;
+        const double sc = abstol_of(c, e) + reltol * fabs(un[e]);
+        tot += (G / sc) * (G / sc);
+    }
+    partial[i] = tot;
+}
+
+// One block: out[0] = sum of v[0..n) (fixed order), out[1] = the slip-rate failure flag, which is
+// cleared for the next evaluation.
+__global__ __launch_bounds__(256) void k_imex_sum(const double* __restrict__ v, int n, int* __restrict__ fail,
+                                                   double* __restrict__ out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) acc += v[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0];
+        out[1] = (double)fail[0];
+        fail[0] = 0;
+    }
+}
+
+// The converged stage derivative F = (U - base) / hg (the stage equation solved for f).
+__global__ void k_imex_stage_f(int ns, const double* __restrict__ U, const double* __restrict__ base, double hg,
+                               double* __restrict__ Fs) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= ns) return;
+    Fs[e] = (U[e] - base[e]) / hg;
+}
+
+// Squared scaled error of the step, per entry: u_new = U (stiffly accurate) against the embedded
+// u_n + h sum bh_j F_j, scale = abstol + reltol max(|u_n|, |u_new|).
+__global__ void k_imex_err(Dev d, const double* __restrict__ un, const double* __restrict__ U,
+                           const double* __restrict__ F, double h, double reltol, double* __restrict__ out) {
+    const int ns = d.c.nstate;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= ns) return;
+    double acc = 0.0;
+    for (int j = 0; j < 5; ++j) acc += kSdBh[j] * F[(size_t)j * ns + e];
+    const double est = U[e] - (un[e] + h * acc);
+    const double sc = abstol_of(d.c, e) + reltol * fmax(fabs(un[e]), fabs(U[e]));
+    const double r = est / sc;
+    out[e] = r * r;
+}
+
 }  // namespace bp8
 }  // namespace eqd
 
@@ -1101,6 +1276,7 @@ struct eqd_bp8 {
     DevMem KR, Tk, ks, Lx, src, active, V2, V3, Vm, tau2, tau3, Vprev, sig_low, floor_hits, floor_mask, fail;
     int ks_split = 0;  // source-row chunks of the Toeplitz K s (0: dense K)
     DevMem k, uprev, unew, ya, yb, partial, ctl, saves, save_t, scratch;
+    DevMem Kd, imex_out;  // the implicit-local integrator: per-node 2 x 2 K blocks, reduction results
     int64_t save_cap = 0, nsaved = 0;
     hipStream_t st = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1379,24 +1555,10 @@ void build_graph(eqd_bp8& m) {
     HIPCHK(hipGraphDestroy(g));
 }
 
-void integrate(eqd_bp8& m, double t0, double tf, double saveat, double reltol, int64_t maxiters, int64_t nsave_max,
-               double* ts, double* us, eqd_bp8_stats* stats) {
-    if (!(tf > t0)) throw Error(EQD_ERR_ARG, "tspan must have tf > t0");
-    if (!(saveat > 0)) throw Error(EQD_ERR_ARG, "saveat must be positive");
-    if (!(reltol > 0)) throw Error(EQD_ERR_ARG, "reltol must be positive");
-    if (nsave_max < 1) throw Error(EQD_ERR_ARG, "nsave_max must be at least 1");
-    if (maxiters <= 0) maxiters = 100000;
+// u_n = u0 on the device (uprev), f(u0) in k[0], and Hairer's initial step (OrdinaryDiffEq
+// ode_determine_initdt, order + 1 = 5).
+double start_state(eqd_bp8& m, double t0, double tf, double reltol) {
     const int ns = m.c.nstate;
-    if (nsave_max > m.save_cap) {
-        m.saves.alloc((size_t)nsave_max * m.ldS() * 8);
-        m.save_t.alloc((size_t)nsave_max * 8);
-        m.save_cap = nsave_max;
-        if (m.exec) {
-            HIPCHK(hipGraphExecDestroy(m.exec));
-            m.exec = nullptr;
-        }
-    }
-    m.nsaved = 0;
     // u0, k1 = f(u0), Hairer's initial step (OrdinaryDiffEq ode_determine_initdt)
     std::vector<double> u0(ns), f0(ns), f1(ns), u1(ns), tmp(ns), abst(ns);
     initial_state(m, u0.data());
@@ -1423,7 +1585,29 @@ void integrate(eqd_bp8& m, double t0, double tf, double saveat, double reltol, i
     const double d2 = rms(tmp) / dt0;
     const double md = std::max(d1, d2);
     const double dt1 = md <= 1e-15 ? std::max(1e-6, dt0 * 1e-3) : std::pow(10.0, -(2 + std::log10(md)) / 5.0);
-    double dt = std::min(std::min(100 * dt0, dt1), dtmax);
+    return std::min(std::min(100 * dt0, dt1), dtmax);
+}
+
+
+void integrate(eqd_bp8& m, double t0, double tf, double saveat, double reltol, int64_t maxiters, int64_t nsave_max,
+               double* ts, double* us, eqd_bp8_stats* stats) {
+    if (!(tf > t0)) throw Error(EQD_ERR_ARG, "tspan must have tf > t0");
+    if (!(saveat > 0)) throw Error(EQD_ERR_ARG, "saveat must be positive");
+    if (!(reltol > 0)) throw Error(EQD_ERR_ARG, "reltol must be positive");
+    if (nsave_max < 1) throw Error(EQD_ERR_ARG, "nsave_max must be at least 1");
+    if (maxiters <= 0) maxiters = 100000;
+    const int ns = m.c.nstate;
+    if (nsave_max > m.save_cap) {
+        m.saves.alloc((size_t)nsave_max * m.ldS() * 8);
+        m.save_t.alloc((size_t)nsave_max * 8);
+        m.save_cap = nsave_max;
+        if (m.exec) {
+            HIPCHK(hipGraphExecDestroy(m.exec));
+            m.exec = nullptr;
+        }
+    }
+    m.nsaved = 0;
+    double dt = start_state(m, t0, tf, reltol);
 
     Ctl& c = *m.h_ctl;
     std::memset(&c, 0, sizeof(Ctl));
@@ -1483,6 +1667,155 @@ void integrate(eqd_bp8& m, double t0, double tf, double saveat, double reltol, i
     if (ts) HIPCHK(hipMemcpy(ts, m.save_t.p, (size_t)r.nsaved * 8, hipMemcpyDeviceToHost));
     if (us)
         HIPCHK(hipMemcpy2D(us, (size_t)ns * 8, m.saves.p, (size_t)m.ldS() * 8, (size_t)ns * 8, (size_t)r.nsaved,
+                           hipMemcpyDeviceToHost));
+}
+
+// The implicit-local integrator (see k_imex_newton): SDIRK stages solved by per-node block Newton,
+// the step controlled by the embedded order-3 estimate, steps ending on every save time and on
+// t_off (so saves are exact step ends, no interpolation). Newton converges when the RMS of the
+// scaled update is below kNewtonTol; a stage that has not converged after kNewtonMax iterations
+// (or whose slip-rate solve failed) rejects the step with dt / 4.
+constexpr double kNewtonTol = 0.03;
+constexpr int kNewtonMax = 10;
+
+void integrate_imex(eqd_bp8& m, double t0, double tf, double saveat, double reltol, int64_t maxiters,
+                    int64_t nsave_max, double* ts, double* us, eqd_bp8_stats* stats) {
+    if (!(tf > t0)) throw Error(EQD_ERR_ARG, "tspan must have tf > t0");
+    if (!(saveat > 0)) throw Error(EQD_ERR_ARG, "saveat must be positive");
+    if (!(reltol > 0)) throw Error(EQD_ERR_ARG, "reltol must be positive");
+    if (nsave_max < 1) throw Error(EQD_ERR_ARG, "nsave_max must be at least 1");
+    if (maxiters <= 0) maxiters = 100000;
+    const int ns = m.c.nstate, nf = m.c.nf;
+    if (nsave_max > m.save_cap) {
+        m.saves.alloc((size_t)nsave_max * m.ldS() * 8);
+        m.save_t.alloc((size_t)nsave_max * 8);
+        m.save_cap = nsave_max;
+        if (m.exec) {
+            HIPCHK(hipGraphExecDestroy(m.exec));
+            m.exec = nullptr;
+        }
+    }
+    if (!m.Kd.p) {
+        m.Kd.alloc((size_t)4 * nf * 8);
+        m.imex_out.alloc(2 * 8);
+        hipLaunchKernelGGL(k_imex_kdiag, dim3((nf + 255) / 256), dim3(256), 0, m.st, m.dev(), m.Kd.as<double>());
+        HIPCHK(hipGetLastError());
+    }
+    m.nsaved = 0;
+    double dt = start_state(m, t0, tf, reltol);  // u_n in uprev, f(u_n) in k[0]
+    Dev d = m.dev();
+    double* F = m.k.as<double>();                   // stage derivatives F_0 .. F_4 in k[0..4]
+    double* Fn = F + (size_t)5 * ns;                // f(u_n) (the first stage's Newton start)
+    double* Ft = F + (size_t)6 * ns;                // f(U) of the current Newton iterate
+    double* U = m.ya.as<double>();
+    double* base = m.yb.as<double>();
+    double* un = m.uprev.as<double>();
+    double* ev = m.scratch.as<double>();
+    HIPCHK(hipMemcpyAsync(Fn, F, (size_t)ns * 8, hipMemcpyDeviceToDevice, m.st));
+    HIPCHK(hipMemcpyAsync(m.saves.p, un, (size_t)ns * 8, hipMemcpyDeviceToDevice, m.st));
+    HIPCHK(hipMemcpyAsync(m.save_t.p, &t0, 8, hipMemcpyHostToDevice, m.st));
+    int64_t nsaved = 1, naccept = 0, nreject = 0, nrhs = 2, iters = 0;
+    int retcode = 0;
+    double t = t0;
+    const double sc_t = 1e-9 * saveat;
+    double next_save = t0 + saveat;
+    const int sb = m.state_blocks(), nb = (nf + 255) / 256;
+    double red[2];
+    auto reduce = [&](const double* v, int n) {
+        hipLaunchKernelGGL(k_imex_sum, dim3(1), dim3(256), 0, m.st, v, n, d.fail, m.imex_out.as<double>());
+        HIPCHK(hipMemcpyAsync(red, m.imex_out.p, 16, hipMemcpyDeviceToHost, m.st));
+        HIPCHK(hipStreamSynchronize(m.st));
+    };
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, m.st));
+    while (t < tf) {
+        if (iters >= maxiters) {
+            retcode = 1;
+            break;
+        }
+        ++iters;
+        // the step ends on the next save time, on t_off and on tf when it would pass them
+        double stop = std::min(tf, next_save);
+        if (m.c.t_off > t && m.c.t_off < stop) stop = m.c.t_off;
+        const bool clipped = dt >= stop - t;
+        const double h = clipped ? stop - t : dt;
+        const double hg = 0.25 * h;
+        static const double cs[5] = {0.25, 0.75, 11.0 / 20.0, 0.5, 1.0};
+        bool ok = true;
+        for (int st = 0; st < 5 && ok; ++st) {
+            const double* Fg = st == 0 ? Fn : F + (size_t)(st - 1) * ns;
+            hipLaunchKernelGGL(k_imex_base, dim3(sb), dim3(256), 0, m.st, ns, st, h, un, F, Fg, base, U);
+            bool conv = false;
+            for (int it = 0; it < kNewtonMax; ++it) {
+                toeplitz_ks(m, d, 0, U);
+                stage_launch(m, d, 0, U, Ft, (double*)nullptr, t + cs[st] * h);
+                hipLaunchKernelGGL(k_imex_newton, dim3(nb), dim3(256), 0, m.st, d, m.Kd.as<double>(), Ft, base, hg,
+                                   un, reltol, U, m.partial.as<double>());
+                HIPCHK(hipGetLastError());
+                reduce(m.partial.as<double>(), nf);
+                ++nrhs;
+                if (red[1] != 0.0 || !std::isfinite(red[0])) break;  // slip-rate solve failed
+                if (std::sqrt(red[0] / ns) < kNewtonTol) {
+                    conv = true;
+                    break;
+                }
+            }
+            if (!conv) {
+                ok = false;
+                break;
+            }
+            hipLaunchKernelGGL(k_imex_stage_f, dim3(sb), dim3(256), 0, m.st, ns, U, base, hg, F + (size_t)st * ns);
+        }
+        double err = 0.0;
+        if (ok) {
+            hipLaunchKernelGGL(k_imex_err, dim3(sb), dim3(256), 0, m.st, d, un, U, F, h, reltol, ev);
+            reduce(ev, ns);
+            err = std::sqrt(red[0] / ns);
+            ok = std::isfinite(err);
+        }
+        if (ok && err <= 1.0) {
+            t = clipped ? stop : t + h;
+            ++naccept;
+            HIPCHK(hipMemcpyAsync(un, U, (size_t)ns * 8, hipMemcpyDeviceToDevice, m.st));
+            HIPCHK(hipMemcpyAsync(Fn, F + (size_t)4 * ns, (size_t)ns * 8, hipMemcpyDeviceToDevice, m.st));
+            const bool on_save = std::fabs(t - next_save) <= sc_t;
+            const bool at_end = t >= tf;
+            if ((on_save || at_end) && nsaved < nsave_max) {
+                HIPCHK(hipMemcpyAsync(m.saves.as<double>() + (size_t)nsaved * m.ldS(), un, (size_t)ns * 8,
+                                      hipMemcpyDeviceToDevice, m.st));
+                const double tsv = on_save ? next_save : tf;
+                HIPCHK(hipMemcpy(m.save_t.as<double>() + nsaved, &tsv, 8, hipMemcpyHostToDevice));
+                ++nsaved;
+            }
+            if (on_save) next_save = t0 + (double)nsaved * saveat;
+            const double fac = std::min(4.0, std::max(0.2, 0.9 * std::pow(std::max(err, 1e-10), -0.25)));
+            dt = clipped ? std::max(dt, h * fac) : h * fac;
+        } else {
+            ++nreject;
+            dt = ok ? h * std::max(0.2, 0.9 * std::pow(err, -0.25)) : h * 0.25;
+            if (dt < 1e-12 * std::max(1.0, std::fabs(t))) throw Error(EQD_ERR_NEWTON, "implicit stage solve failed at the minimum step");
+        }
+    }
+    HIPCHK(hipEventRecord(e1, m.st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    m.nsaved = nsaved;
+    if (stats) {
+        stats->naccept = naccept;
+        stats->nreject = nreject;
+        stats->nrhs = nrhs;
+        stats->nsaved = nsaved;
+        stats->retcode = retcode;
+        stats->ms = ms;
+    }
+    if (ts) HIPCHK(hipMemcpy(ts, m.save_t.p, (size_t)nsaved * 8, hipMemcpyDeviceToHost));
+    if (us)
+        HIPCHK(hipMemcpy2D(us, (size_t)ns * 8, m.saves.p, (size_t)m.ldS() * 8, (size_t)ns * 8, (size_t)nsaved,
                            hipMemcpyDeviceToHost));
 }
 
@@ -1867,6 +2200,14 @@ eqd_status eqd_bp8_integrate(eqd_bp8* m, double t0, double tf, double saveat, do
     return guard([&] {
         if (!m) throw Error(EQD_ERR_ARG, "null model");
         integrate(*m, t0, tf, saveat, reltol, maxiters, nsave_max, ts, us, stats);
+    });
+}
+
+eqd_status eqd_bp8_integrate_imex(eqd_bp8* m, double t0, double tf, double saveat, double reltol, int64_t maxiters,
+                                  int64_t nsave_max, double* ts, double* us, eqd_bp8_stats* stats) {
+    return guard([&] {
+        if (!m) throw Error(EQD_ERR_ARG, "null model");
+        integrate_imex(*m, t0, tf, saveat, reltol, maxiters, nsave_max, ts, us, stats);
     });
 }
 

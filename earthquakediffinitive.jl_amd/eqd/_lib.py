@@ -30,7 +30,7 @@ EXPORTED = [
     "eqd_tables_export", "eqd_version", "eqd_last_error",
     "eqd_bp8_default_params", "eqd_bp8_create", "eqd_bp8_create_device", "eqd_bp8_create_from_solver",
     "eqd_bp8_destroy", "eqd_bp8_sizes", "eqd_bp8_stiffness", "eqd_bp8_operators", "eqd_bp8_initial_state",
-    "eqd_bp8_evaluate", "eqd_bp8_rhs", "eqd_bp8_stress_report", "eqd_bp8_reset_cache", "eqd_bp8_integrate",
+    "eqd_bp8_evaluate", "eqd_bp8_rhs", "eqd_bp8_stress_report", "eqd_bp8_reset_cache", "eqd_bp8_integrate", "eqd_bp8_integrate_imex",
     "eqd_bp8_saved_device", "eqd_bp8_saved_vmax", "eqd_bp8_saved_outputs",
     "eqd_server_create", "eqd_server_destroy", "eqd_server_add_grid", "eqd_server_submit_columns",
     "eqd_server_submit_forward", "eqd_server_step", "eqd_server_completed", "eqd_server_result",
@@ -138,6 +138,7 @@ def lib():
         L.eqd_bp8_reset_cache.argtypes = [vp]
         L.eqd_bp8_integrate.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int64,
                                         vp, vp, C.POINTER(BP8Stats)]
+        L.eqd_bp8_integrate_imex.argtypes = L.eqd_bp8_integrate.argtypes
         L.eqd_bp8_saved_device.argtypes = [vp, C.POINTER(vp), i64p, i64p]
         L.eqd_bp8_saved_vmax.argtypes = [vp, dp, C.c_int64]
         L.eqd_bp8_saved_outputs.argtypes = [vp, C.c_int64, vp, C.c_int, vp, vp]

@@ -211,15 +211,20 @@ class Solution:
 
 
 def run_bp8(m: BP8Model, *, tspan=None, reltol: float = 1e-8, saveat: float = 3600.0, verbose: bool = False,
-            maxiters: int = 100000) -> Solution:
-    """run_bp8(m; tspan, reltol, saveat) (src/BP8.jl:442-470): Tsit5 on the device."""
+            maxiters: int = 100000, alg: str = "tsit5") -> Solution:
+    """run_bp8(m; tspan, reltol, saveat, alg) (src/BP8.jl:442-470) on the device. alg = "tsit5" (the
+    reference's default) or "imex": the implicit-local SDIRK for the stiff Peaceman-well variant
+    (eqd_bp8_integrate_imex)."""
+    if alg not in ("tsit5", "imex"):
+        raise EqdError(_lib.EQD_ERR_ARG, f"alg must be tsit5 or imex, got {alg}")
     t0, tf = (0.0, m.par["t_f"]) if tspan is None else (float(tspan[0]), float(tspan[1]))
     nmax = int(math.floor((tf - t0) / saveat + 1e-9)) + 2
     ts = np.zeros(nmax)
     us = np.zeros(nmax * m.state_len)
     st = _lib.BP8Stats()
-    check(lib().eqd_bp8_integrate(m.handle, t0, tf, float(saveat), float(reltol), int(maxiters), nmax,
-                                  ts.ctypes.data, us.ctypes.data, C.byref(st)))
+    fn = lib().eqd_bp8_integrate if alg == "tsit5" else lib().eqd_bp8_integrate_imex
+    check(fn(m.handle, t0, tf, float(saveat), float(reltol), int(maxiters), nmax, ts.ctypes.data, us.ctypes.data,
+             C.byref(st)))
     k = st.nsaved
     return Solution(ts[:k].copy(), us[:k * m.state_len].reshape(k, m.state_len).copy(), st.naccept, st.nreject,
                     st.nrhs, "Success" if st.retcode == 0 else "MaxIters", st.ms)

@@ -240,6 +240,18 @@ typedef struct {
  * maxiters <= 0 means 100000 (OrdinaryDiffEq's default). */
 eqd_status eqd_bp8_integrate(eqd_bp8* m, double t0, double tf, double saveat, double reltol, int64_t maxiters,
                              int64_t nsave_max, double* ts, double* us, eqd_bp8_stats* stats);
+/* run_bp8 with an implicit-local integrator instead of Tsit5, for the stiff Peaceman-well
+ * variant (no counterpart in the reference, which scopes it in PROGRESS.md:872-894 and
+ * TODO.md:126-153: the stiff mode is the well node's slip against its own self-stiffness, local
+ * to one node). The 5-stage L-stable SDIRK of order 4 (Hairer & Wanner, gamma = 1/4) with its
+ * embedded order-3 error estimate; each stage equation is solved by Newton iterations whose
+ * Jacobian is the per-node 3 x 3 block of (s2, s3, ln theta), the dense K coupling and the pore
+ * pressure converging as a fixed point. Steps end on every save time, on t_off and on tf (saves
+ * are step ends). Same arguments, tolerances (reltol; abstol 1e-14 / 1e-10 / 1e-3 per block) and
+ * saves as eqd_bp8_integrate; stats->nrhs counts right-hand-side evaluations. */
+eqd_status eqd_bp8_integrate_imex(eqd_bp8* m, double t0, double tf, double saveat, double reltol,
+                                  int64_t maxiters, int64_t nsave_max, double* ts, double* us,
+                                  eqd_bp8_stats* stats);
 /* Device pointer to the saved states of the last integrate: row j (save j) starts at
  * us_dev + j * ld (ld >= state_len, even). */
 eqd_status eqd_bp8_saved_device(const eqd_bp8* m, const double** us_dev, int64_t* ld, int64_t* nsaved);
