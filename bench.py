@@ -288,11 +288,14 @@ def bench_production(args):
 
 
 def bench_ensemble(args):
-    """Trajectory ensemble on MFMA: dz = 20 m, L_fault = 800, L_normal = 400, l_f = 800 (6561 fault
-    nodes, K 13122 x 13122 FP64 = 1.38 GB from the device :toeplitz build), 256 members with seeded
-    perturbations of Q0 (+-20%), a - b (+-10%) and sigma0 (+-5%), Gaussian source, integrated
-    over args.ens_hours (default 30 days) with saveat 3600 s. K S per stage is one FP64 MFMA GEMM (2 * 13122^2 *
-    256 = 88.2 GFLOP); its roofline is the MI355X dense FP64 matrix peak."""
+    """Trajectory ensemble: dz = 20 m, L_fault = 800, L_normal = 400, l_f = 800 (6561 fault nodes, K
+    13122 x 13122 FP64 = 1.38 GB from the device :toeplitz build), 256 members with seeded
+    perturbations of Q0 (+-20%), a - b (+-10%) and sigma0 (+-5%), Gaussian source, integrated over
+    args.ens_hours (default 30 days) with saveat 3600 s. K S per stage is one 2D FFT convolution
+    batched over the members (transform 256 x 256; its roofline is HBM, by the bytes of the
+    transform planes it writes and reads back). The Peaceman-well members run on a model of the same
+    K handed over dense, whose K S is one FP64 MFMA GEMM (2 * 13122^2 * 256 = 88.2 GFLOP, roofline
+    the dense FP64 matrix peak): both products are timed."""
     from eqd import bp8
     from eqd.ensemble import Ensemble, perturbed_members
     par = bp8.benchmark_parameters(l_f=800.0, dz=20.0)
@@ -303,12 +306,21 @@ def bench_ensemble(args):
     # model) and from the dense 1.38 GB K (the same K handed to model_from_stiffness)
     single = {}
     Kh = m.K
-    for name, mm in (("toeplitz_kernel", m), ("dense_K", bp8.model_from_stiffness(Kh, m.n2f, par=m.par))):
+    ks_us = {}
+    for meth in ("direct", "fft"):
+        m.set_ks_method(meth)
+        ks_us[meth] = round(m.ks_timing(200) * 1e3, 2)
+    for name, mm, meth in (("toeplitz_fft", m, "fft"), ("toeplitz_direct", m, "direct"),
+                           ("dense_K", bp8.model_from_stiffness(Kh, m.n2f, par=m.par), None)):
+        if meth:
+            mm.set_ks_method(meth)
         sol = bp8.run_bp8(mm, tspan=(0.0, args.ens_hours * 3600.0), saveat=3600.0)
         single[name] = {"device_loop_ms": round(sol.ms, 1), "rhs_evaluations": sol.nrhs,
                         "us_per_rhs": round(sol.ms * 1e3 / max(1, sol.nrhs), 2), "accepted_steps": sol.naccept,
                         "final_slip_centre": float(sol.u[-1, (m.n2f // 2) * (m.n2f + 1)])}
         del mm
+    m.set_ks_method("fft")
+    single["ks_us_per_product"] = ks_us
     mem = perturbed_members(m.par, args.ens_members, seed=20261203)
     e = Ensemble(m, args.ens_members, **mem)
     t0 = time.perf_counter()
@@ -331,8 +343,12 @@ def bench_ensemble(args):
             worst = max(worst, float(np.max(np.abs(ens_v[j] - ro["vmax"]) / ro["vmax"])))
         cpu_check = {"members": [0, 1], "hours": 2, "vmax_max_rel_diff_vs_oracle": worst,
                      "cpu_ms": round((time.perf_counter() - t0) * 1e3, 1)}
-    ms_gemm, flops = e.gemm_timing(20)
-    tf = flops / (ms_gemm * 1e-3)
+    ms_ks, _ = e.gemm_timing(20)
+    # the transform planes (rows x M complex, written and read back twice) plus the slips read and
+    # the tractions written, per member
+    Mt = 1 << (2 * m.n2f - 2).bit_length()
+    plane_bytes = args.ens_members * (4 * m.n2f * Mt * 16 + 2 * 2 * m.nf * 8)
+    fft_bw = plane_bytes / (ms_ks * 1e-3)
     acc = [s.naccept for s in st]
     ens_loop_ms = e.loop_ms
     del e
@@ -340,6 +356,7 @@ def bench_ensemble(args):
     # as the well pressure builds (104 in the first 3 h, ~1e6 per 100 h at this spacing by the
     # reference's dz^-4 scaling of its 31,249 steps per 100 h at dz = 50 m)
     pw = None
+    gemm = None
     if args.ens_pw_hours > 0:
         mpw = bp8.model_from_stiffness(Kh, m.n2f, par=m.par, injection="peaceman")
         epw = Ensemble(mpw, args.ens_members, **mem)
@@ -351,6 +368,10 @@ def bench_ensemble(args):
         stp = epw.run(tspan=(0.0, args.ens_pw_hours * 3600.0), saveat=600.0, maxiters=args.ens_pw_maxiters)
         t_pw = time.perf_counter() - t0
         accp = [s.naccept for s in stp]
+        ms_gemm, flops = epw.gemm_timing(20)
+        tf = flops / (ms_gemm * 1e-3)
+        gemm = {"bound": "mfma", "achieved": round(tf / 1e12, 2), "peak": 78.6, "unit": "TFLOP/s",
+                "frac": round(tf / 78.6e12, 4), "ms_per_gemm": round(ms_gemm, 3), "flop_per_gemm": flops}
         pw = {"hours": args.ens_pw_hours, "run_ms": round(t_pw * 1e3, 1), "device_loop_ms": round(epw.loop_ms, 1),
               "accepted_steps_min_max": [int(min(accp)), int(max(accp))],
               "member_steps_per_s": round(sum(accp) / (epw.loop_ms * 1e-3), 1),
@@ -391,10 +412,15 @@ def bench_ensemble(args):
            "accepted_steps_min_max": [int(min(acc)), int(max(acc))],
            "member_steps_per_s": round(sum(acc) / (ens_loop_ms * 1e-3), 1), "single_trajectory": single,
            "cpu_member_check": cpu_check, "peaceman": pw, "peaceman_imex": pw_imex,
-           "gemm": {"bound": "mfma", "achieved": round(tf / 1e12, 2), "peak": 78.6, "unit": "TFLOP/s",
-                    "frac": round(tf / 78.6e12, 4), "ms_per_gemm": round(ms_gemm, 3), "flop_per_gemm": flops}}
+           "ks_fft": {"bound": "hbm", "achieved": round(fft_bw / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
+                      "frac": round(fft_bw / 8e12, 4), "ms_per_product": round(ms_ks, 4),
+                      "transform": [Mt, Mt], "bytes_per_product": plane_bytes,
+                      "vs_dense_gemm": round(gemm["ms_per_gemm"] / ms_ks, 2) if gemm else None},
+           "gemm_dense_K": gemm}
     print(f"bench: ensemble dz=20 {args.ens_members} members {args.ens_hours} h: K build {t_build * 1e3:.0f} ms, "
-          f"loop {ens_loop_ms:.0f} ms, GEMM {ms_gemm:.3f} ms = {tf / 1e12:.1f} TFLOP/s", file=sys.stderr, flush=True)
+          f"loop {ens_loop_ms:.0f} ms, FFT K S {ms_ks:.3f} ms"
+          + (f", dense GEMM {gemm['ms_per_gemm']:.3f} ms = {gemm['achieved']:.1f} TFLOP/s" if gemm else ""),
+          file=sys.stderr, flush=True)
     return res
 
 

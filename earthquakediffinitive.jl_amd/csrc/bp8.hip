@@ -25,6 +25,7 @@
 
 #include "../../include/eqd.h"
 #include "eqd_internal.h"
+#include "fft_ks.h"
 #include "host_common.h"
 
 using namespace eqd;
@@ -79,8 +80,8 @@ struct Dev {
     Consts c;
     const double* KR;      // K row-major, 2nf x 2nf
     const double* Tk;      // Toeplitz K: 4 blocks per separation (see k_toeplitz_kernel), or null
-    double* ks;            // with Tk: K s partials, ks_split x nf x 2 (see k_toeplitz_ks)
-    int ks_split;          // source x3 rows are split into this many chunks
+    double* ks;            // with Tk: K s partials, ks_split x nf x 2 (k_toeplitz_ks, or one chunk from the FFT)
+    int ks_split;          // chunks of ks the stage kernel adds (0: it forms K s from Tk itself)
     const double* Lx;      // n2f x lw: band of each row of the 1D pore-pressure factor (alpha not applied)
     const double* src;     // nf
     const unsigned char* active;
@@ -287,7 +288,8 @@ __global__ __launch_bounds__(256) void k_bp8_stage(Dev d, int stage, const doubl
     }
 }
 
-// The stage of a model whose K s comes precomputed per node from k_toeplitz_ks (ks_split > 0):
+// The stage of a model whose K s comes precomputed per node (ks_split > 0: from k_toeplitz_ks or
+// the FFT convolution):
 // one node per thread, where the wave-per-node k_bp8_stage would repeat each node's scalar
 // Newton on 64 lanes. The per-node arithmetic is k_bp8_stage's except the order of two sums
 // (the K s chunks and the A_p row are added here one after another instead of across lanes).
@@ -1274,7 +1276,9 @@ struct eqd_bp8 {
     std::vector<double> Lx_h, src_h, w_h;
     std::vector<unsigned char> active_h;
     DevMem KR, Tk, ks, Lx, src, active, V2, V3, Vm, tau2, tau3, Vprev, sig_low, floor_hits, floor_mask, fail;
-    int ks_split = 0;  // source-row chunks of the Toeplitz K s (0: dense K)
+    int ks_split = 0;  // source-row chunks of the direct Toeplitz K s (0: dense K or in-kernel)
+    int ks_method = EQD_KS_DIRECT;  // Toeplitz K s: direct correlation or FFT convolution
+    FftPlan fft;                    // the FFT convolution's plan (ks_method == EQD_KS_FFT)
     DevMem k, uprev, unew, ya, yb, partial, ctl, saves, save_t, scratch;
     DevMem Kd, imex_out;  // the implicit-local integrator: per-node 2 x 2 K blocks, reduction results
     int64_t save_cap = 0, nsaved = 0;
@@ -1294,7 +1298,7 @@ struct eqd_bp8 {
         d.KR = KR.as<double>();
         d.Tk = Tk.p ? Tk.as<double>() : nullptr;
         d.ks = ks.p ? ks.as<double>() : nullptr;
-        d.ks_split = ks_split;
+        d.ks_split = ks_chunks();
         d.Lx = Lx.as<double>();
         d.src = src.as<double>();
         d.active = active.as<unsigned char>();
@@ -1319,6 +1323,8 @@ struct eqd_bp8 {
         return d;
     }
     int node_blocks() const { return (c.nf + 3) / 4; }
+    // K s chunks the stage kernel adds (0: the stage kernel forms K s itself)
+    int ks_chunks() const { return ks_method == EQD_KS_FFT ? 1 : ks_split; }
     int ldS() const { return (c.nstate + 1) & ~1; }
     int state_blocks() const { return (c.nstate + 255) / 256; }
 };
@@ -1335,6 +1341,8 @@ struct eqd_bp8_ensemble {
     Ctl* h_ctl = nullptr;
     double gemm_flops_per_call = 0;
     int splits = 1, kchunk = 0;  // split-K of the K S GEMM
+    bool use_fft = false;        // Toeplitz model: K S as the batched FFT convolution (no Kc)
+    FftPlan fft;
     ~eqd_bp8_ensemble() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (h_ctl) (void)hipHostFree(h_ctl);
@@ -1487,8 +1495,36 @@ void initial_state(const eqd_bp8& m, double* u) {
     for (int i = 0; i < m.c.nf; ++i) u[2 * m.c.nf + i] = std::log(theta0);
 }
 
+// The host copy of a model's separation kernel (4 (2 n2f - 1)^2 values).
+std::vector<double> separation_kernel(const eqd_bp8& m) {
+    std::vector<double> Tk(m.Tk.bytes / 8);
+    HIPCHK(hipMemcpy(Tk.data(), m.Tk.p, m.Tk.bytes, hipMemcpyDeviceToHost));
+    return Tk;
+}
+
+// Selects how a Toeplitz-kernel model forms K s: EQD_KS_DIRECT (the separation-kernel sum, in the
+// stage kernel up to 48 x 48 nodes, k_toeplitz_ks beyond) or EQD_KS_FFT (the FFT convolution).
+void set_ks_method(eqd_bp8& m, int method) {
+    if (method != EQD_KS_DIRECT && method != EQD_KS_FFT)
+        throw Error(EQD_ERR_ARG, "K s method must be EQD_KS_DIRECT or EQD_KS_FFT, got " + std::to_string(method));
+    if (!m.Tk.p) throw Error(EQD_ERR_ARG, "the model has a dense K (no separation kernel)");
+    if (method == EQD_KS_FFT && !m.fft.M) {
+        fft_plan_build(m.fft, separation_kernel(m), m.c.n2f, 1);
+        if (m.ks.bytes < (size_t)m.c.nf * 2 * 8) m.ks.alloc((size_t)m.c.nf * 2 * 8);
+    }
+    if (method != m.ks_method && m.exec) {
+        (void)hipGraphExecDestroy(m.exec);
+        m.exec = nullptr;
+    }
+    m.ks_method = method;
+}
+
 // K s partials of a Toeplitz-K model for the stage about to run on y (no-op for a dense K).
 void toeplitz_ks(eqd_bp8& m, const Dev& d, int stage, const double* y) {
+    if (m.ks_method == EQD_KS_FFT) {
+        fft_ks_launch(m.fft, m.st, y, 0, m.c.nf, d.ks, 0, 2, 1, 1, d.ctl, stage != 0);
+        return;
+    }
     if (!m.ks_split) return;
     const int n2 = m.c.n2f, R = ks_slots(n2);
     const size_t lds = (size_t)R * (2 * n2 - 1) * 32 + (size_t)R * n2 * 16;
@@ -1497,7 +1533,7 @@ void toeplitz_ks(eqd_bp8& m, const Dev& d, int stage, const double* y) {
 
 // One stage kernel: a thread per node when K s comes precomputed, else a wave per node.
 void stage_launch(eqd_bp8& m, const Dev& d, int stage, const double* y, double* kout, double* ynext, double t) {
-    if (m.ks_split)
+    if (m.ks_chunks())
         hipLaunchKernelGGL(k_bp8_stage_nodes, dim3((m.c.nf + 255) / 256), dim3(256), 0, m.st, d, stage, y, kout,
                            ynext, t);
     else
@@ -1819,9 +1855,17 @@ void integrate_imex(eqd_bp8& m, double t0, double tf, double saveat, double relt
                            hipMemcpyDeviceToHost));
 }
 
-// K S over the members' slip vectors (rows 0 .. 2nf-1 of y, member stride ldY) into E.D.
-void ens_gemm(eqd_bp8_ensemble& E, const double* y, int64_t ldY) {
+// K S over the members' slip vectors (rows 0 .. 2nf-1 of y, member stride ldY) into E.D: the
+// batched FFT convolution for a Toeplitz model (members already done skipped when skip_done),
+// else the MFMA GEMM with the dense K.
+void ens_gemm(eqd_bp8_ensemble& E, const double* y, int64_t ldY, bool skip_done = false) {
     const int n2 = E.base->c.n2;
+    if (E.use_fft) {
+        const int nf = E.base->c.nf;
+        fft_ks_launch(E.fft, E.st, y, ldY, nf, E.D.as<double>(), (int64_t)n2, 1, (int64_t)nf, E.M, E.ctl.as<Ctl>(),
+                      skip_done ? 1 : 0);
+        return;
+    }
     hipLaunchKernelGGL(k_ks_gemm, dim3((n2 + kGemmBM - 1) / kGemmBM, (E.M + kGemmBN - 1) / kGemmBN, E.splits),
                        dim3(256), 0, E.st, n2, E.M, n2, E.Kc.as<double>(), (int64_t)n2, y, ldY, E.D.as<double>(),
                        (int64_t)n2, E.kchunk);
@@ -1963,7 +2007,7 @@ void ens_integrate(eqd_bp8_ensemble& E, double t0, double tf, double saveat, dou
             const double* in[8] = {nullptr, nullptr, ya, yb, ya, yb, ya, un};
             double* out[8] = {nullptr, nullptr, yb, ya, yb, ya, un, nullptr};
             for (int stage = 2; stage <= 7; ++stage) {
-                ens_gemm(E, in[stage], ld);
+                ens_gemm(E, in[stage], ld, true);
                 ens_stage(E, stage, in[stage], ld, kk + (size_t)(stage - 1) * kst, out[stage]);
             }
             hipLaunchKernelGGL(k_ens_control, dim3((M + 63) / 64), dim3(64), 0, E.st, E.dev(), E.node_blocks());
@@ -2084,8 +2128,70 @@ eqd_status eqd_bp8_create_from_solver(eqd_solver* s, int stiffness, const eqd_bp
                                (long long)(2 * nf), n2f, m->Tk.as<double>());
             HIPCHK(hipGetLastError());
             HIPCHK(hipStreamSynchronize(m->st));
+            // beyond 48 x 48 nodes K s is the FFT convolution by default (up to 256 x 256)
+            if (m->ks_split && FftPlan::size_for(n2f) <= kFftMaxM) set_ks_method(*m, EQD_KS_FFT);
         }
         *out = m.release();
+    });
+}
+
+eqd_status eqd_bp8_set_ks_method(eqd_bp8* m, int method) {
+    return guard([&] {
+        if (!m) throw Error(EQD_ERR_ARG, "null model");
+        set_ks_method(*m, method);
+    });
+}
+
+eqd_status eqd_bp8_ks(eqd_bp8* m, const double* s, double* out) {
+    return guard([&] {
+        if (!m || !s || !out) throw Error(EQD_ERR_ARG, "null argument");
+        if (!m->Tk.p) throw Error(EQD_ERR_ARG, "the model has a dense K (no separation kernel)");
+        if (!m->ks_chunks()) throw Error(EQD_ERR_ARG, "the model forms K s inside its stage kernel (EQD_KS_DIRECT at "
+                                                     "48 x 48 nodes or fewer)");
+        const int nf = m->c.nf, chunks = m->ks_chunks();
+        HIPCHK(hipMemcpyAsync(m->ya.p, s, (size_t)2 * nf * 8, hipMemcpyHostToDevice, m->st));
+        toeplitz_ks(*m, m->dev(), 0, m->ya.as<double>());
+        HIPCHK(hipGetLastError());
+        std::vector<double> part((size_t)chunks * nf * 2);
+        HIPCHK(hipMemcpyAsync(part.data(), m->ks.p, part.size() * 8, hipMemcpyDeviceToHost, m->st));
+        HIPCHK(hipStreamSynchronize(m->st));
+        // the chunks added in the order the stage kernel adds them
+        for (int i = 0; i < nf; ++i) {
+            double a2 = 0.0, a3 = 0.0;
+            for (int q = 0; q < chunks; ++q) {
+                a2 += part[((size_t)q * nf + i) * 2];
+                a3 += part[((size_t)q * nf + i) * 2 + 1];
+            }
+            out[i] = a2;
+            out[nf + i] = a3;
+        }
+    });
+}
+
+eqd_status eqd_bp8_ks_timing(eqd_bp8* m, int reps, double* ms_per_product) {
+    return guard([&] {
+        if (!m || reps < 1) throw Error(EQD_ERR_ARG, "null model or reps < 1");
+        if (!m->Tk.p) throw Error(EQD_ERR_ARG, "the model has a dense K (no separation kernel)");
+        if (!m->ks_chunks()) throw Error(EQD_ERR_ARG, "the model forms K s inside its stage kernel (EQD_KS_DIRECT "
+                                                     "at 48 x 48 nodes or fewer): nothing to time");
+        std::vector<double> u0(m->c.nstate);
+        initial_state(*m, u0.data());
+        HIPCHK(hipMemcpyAsync(m->ya.p, u0.data(), u0.size() * 8, hipMemcpyHostToDevice, m->st));
+        Dev d = m->dev();
+        toeplitz_ks(*m, d, 0, m->ya.as<double>());
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, m->st));
+        for (int r = 0; r < reps; ++r) toeplitz_ks(*m, d, 0, m->ya.as<double>());
+        HIPCHK(hipEventRecord(e1, m->st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIPCHK(hipGetLastError());
+        if (ms_per_product) *ms_per_product = ms / reps;
     });
 }
 
@@ -2304,19 +2410,27 @@ eqd_status eqd_bp8_ensemble_create(eqd_bp8* base, int members, const double* Q0,
         E->fail.alloc(M * sizeof(int));
         E->k.alloc(7 * ld * M * 8);
         for (DevMem* v : {&E->uprev, &E->unew, &E->ya, &E->yb}) v->alloc(ld * M * 8);
-        // K column-major (the model keeps it row-major for its GEMV); split-K so that the GEMM
-        // has about 12 blocks per CU (at most 8 splits), with at least 64 k per split
         const int n2 = base->c.n2;
-        E->Kc.alloc((size_t)n2 * n2 * 8);
-        hipLaunchKernelGGL(k_transpose, dim3((n2 + 31) / 32, (n2 + 31) / 32), dim3(32, 8), 0, E->st,
-                           base->KR.as<double>(), (long long)n2, n2, E->Kc.as<double>());
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(E->st));
-        const int tiles = ((n2 + kGemmBM - 1) / kGemmBM) * ((members + kGemmBN - 1) / kGemmBN);
-        int want = std::max(1, std::min(8, (3072 + tiles - 1) / tiles));
-        want = std::max(1, std::min(want, n2 / 64));
-        E->kchunk = ((n2 + want - 1) / want + kGemmBK - 1) / kGemmBK * kGemmBK;
-        E->splits = (n2 + E->kchunk - 1) / E->kchunk;
+        if (base->Tk.p && FftPlan::size_for(base->c.n2f) <= kFftMaxM) {
+            // Toeplitz model: K S as the batched FFT convolution from the separation kernel, no
+            // copy of K
+            E->use_fft = true;
+            fft_plan_build(E->fft, separation_kernel(*base), base->c.n2f, members);
+            E->splits = 1;
+        } else {
+            // K column-major (the model keeps it row-major for its GEMV); split-K so that the GEMM
+            // has about 12 blocks per CU (at most 8 splits), with at least 64 k per split
+            E->Kc.alloc((size_t)n2 * n2 * 8);
+            hipLaunchKernelGGL(k_transpose, dim3((n2 + 31) / 32, (n2 + 31) / 32), dim3(32, 8), 0, E->st,
+                               base->KR.as<double>(), (long long)n2, n2, E->Kc.as<double>());
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(E->st));
+            const int tiles = ((n2 + kGemmBM - 1) / kGemmBM) * ((members + kGemmBN - 1) / kGemmBN);
+            int want = std::max(1, std::min(8, (3072 + tiles - 1) / tiles));
+            want = std::max(1, std::min(want, n2 / 64));
+            E->kchunk = ((n2 + want - 1) / want + kGemmBK - 1) / kGemmBK * kGemmBK;
+            E->splits = (n2 + E->kchunk - 1) / E->kchunk;
+        }
         E->D.alloc((size_t)E->splits * n2 * M * 8);
         E->partial.alloc((size_t)E->node_blocks() * M * 8);
         E->ctl.alloc(M * sizeof(Ctl));
@@ -2332,7 +2446,7 @@ eqd_status eqd_bp8_ensemble_create(eqd_bp8* base, int members, const double* Q0,
         HIPCHK(hipMemset(E->fail.p, 0, M * sizeof(int)));
         std::vector<double> vp(nf * M, 1e-12);
         HIPCHK(hipMemcpy(E->Vprev.p, vp.data(), vp.size() * 8, hipMemcpyHostToDevice));
-        E->gemm_flops_per_call = 2.0 * (double)base->c.n2 * base->c.n2 * members;
+        E->gemm_flops_per_call = E->use_fft ? 0.0 : 2.0 * (double)base->c.n2 * base->c.n2 * members;
         *out = E.release();
     });
 }

@@ -19,6 +19,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 PRECOND_NONE, PRECOND_JACOBI = 0, 1
 ENGINE_BATCHED, ENGINE_PLAIN = 0, 1
 STIFFNESS_EXACT, STIFFNESS_TOEPLITZ = 0, 1
+KS_DIRECT, KS_FFT = 0, 1
 
 # Every symbol include/eqd.h declares (checked by tests/test_capi_symbols.py).
 EXPORTED = [
@@ -31,7 +32,8 @@ EXPORTED = [
     "eqd_bp8_default_params", "eqd_bp8_create", "eqd_bp8_create_device", "eqd_bp8_create_from_solver",
     "eqd_bp8_destroy", "eqd_bp8_sizes", "eqd_bp8_stiffness", "eqd_bp8_operators", "eqd_bp8_initial_state",
     "eqd_bp8_evaluate", "eqd_bp8_rhs", "eqd_bp8_stress_report", "eqd_bp8_reset_cache", "eqd_bp8_integrate", "eqd_bp8_integrate_imex",
-    "eqd_bp8_saved_device", "eqd_bp8_saved_vmax", "eqd_bp8_saved_outputs",
+    "eqd_bp8_saved_device", "eqd_bp8_saved_vmax", "eqd_bp8_saved_outputs", "eqd_bp8_set_ks_method",
+    "eqd_bp8_ks", "eqd_bp8_ks_timing",
     "eqd_server_create", "eqd_server_destroy", "eqd_server_add_grid", "eqd_server_submit_columns",
     "eqd_server_submit_forward", "eqd_server_step", "eqd_server_completed", "eqd_server_result",
     "eqd_server_grid_info",
@@ -143,6 +145,9 @@ def lib():
         L.eqd_bp8_saved_vmax.argtypes = [vp, dp, C.c_int64]
         L.eqd_bp8_saved_outputs.argtypes = [vp, C.c_int64, vp, C.c_int, vp, vp]
         pd = C.POINTER(C.c_double)
+        L.eqd_bp8_set_ks_method.argtypes = [vp, C.c_int]
+        L.eqd_bp8_ks.argtypes = [vp, dp, dp]
+        L.eqd_bp8_ks_timing.argtypes = [vp, C.c_int, pd]
         L.eqd_bp8_ensemble_create.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(vp)]
         L.eqd_bp8_ensemble_destroy.argtypes = [vp]
         L.eqd_bp8_ensemble_integrate.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, vp,

@@ -93,6 +93,28 @@ class BP8Model:
         check(lib().eqd_bp8_operators(self.handle, A.ctypes.data, src.ctypes.data, w.ctypes.data))
         return A, src, w
 
+    def set_ks_method(self, method: str):
+        """K s of a stiffness=:toeplitz model: "fft" (the 2D FFT convolution, the default beyond
+        48 x 48 fault nodes) or "direct" (the separation-kernel sum)."""
+        if method not in ("direct", "fft"):
+            raise EqdError(_lib.EQD_ERR_ARG, f"K s method must be 'direct' or 'fft', got {method}")
+        check(lib().eqd_bp8_set_ks_method(self.handle, _lib.KS_FFT if method == "fft" else _lib.KS_DIRECT))
+
+    def ks(self, s) -> np.ndarray:
+        """K s ([tau2; tau3], 2nf) for a slip vector [s2; s3] with the model's K s method."""
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        if s.shape != (2 * self.nf,):
+            raise EqdError(_lib.EQD_ERR_ARG, f"slip must have {2 * self.nf} entries")
+        out = np.zeros(2 * self.nf)
+        check(lib().eqd_bp8_ks(self.handle, s, out))
+        return out
+
+    def ks_timing(self, reps: int = 50) -> float:
+        """Average device ms of one K s with the model's current method."""
+        ms = C.c_double()
+        check(lib().eqd_bp8_ks_timing(self.handle, int(reps), C.byref(ms)))
+        return ms.value
+
     @property
     def Ap(self):
         return self.operators()[0]

@@ -17,7 +17,7 @@ export DeviceCGSolver, DeviceFaultElasticity, split_node_solve, duplicate, merge
        DeviceBP8Model, build_model, initial_state, evaluate!, rhs!, run_bp8, state_length,
        effective_stress_report, saved_outputs, DeviceEnsemble, run_ensemble!, member_solution,
        ensemble_saved_vmax, DeviceServer, add_grid!, submit_columns!, submit_forward!, serve_step!,
-       completed_requests, request_result, jacobi_preconditioner
+       completed_requests, request_result, jacobi_preconditioner, set_ks_method!, ks_product
 
 const libeqd = joinpath(@__DIR__, "..", "libeqd.so")
 
@@ -27,6 +27,7 @@ const EQD_PRECOND = Dict(:none => Cint(0), :jacobi => Cint(1))
 const EQD_ENGINE = Dict(:batched => Cint(0), :plain => Cint(1))
 const EQD_STIFFNESS = Dict(:exact => Cint(0), :toeplitz => Cint(1))
 const EQD_INJECTION = Dict(:gaussian => Cint(0), :peaceman => Cint(1))
+const EQD_KS = Dict(:direct => Cint(0), :fft => Cint(1))
 
 struct GridDesc
     n1::Cint
@@ -310,20 +311,51 @@ function rhs!(du::Vector{Float64}, u::Vector{Float64}, m::DeviceBP8Model, t)
 end
 
 """
-    run_bp8(m; tspan=(0.0, m.par.t_f), reltol=1e-8, saveat=3600.0, maxiters=100000) -> (; t, u, stats)
+    set_ks_method!(m, :fft | :direct)
 
-run_bp8 (src/BP8.jl:442-470): Tsit5 with OrdinaryDiffEq's defaults, stepped on the device.
+How a `stiffness=:toeplitz` model forms K s in rhs! (src/FaultResponse.jl:378-390 makes K a
+function of node separation): the 2D FFT convolution (default beyond 48 x 48 fault nodes) or the
+direct correlation. Errors for a dense-K model.
+"""
+function set_ks_method!(m::DeviceBP8Model, method::Symbol)
+    haskey(EQD_KS, method) || error("K s method must be :fft or :direct, got $method")
+    check(ccall((:eqd_bp8_set_ks_method, libeqd), Cint, (Ptr{Cvoid}, Cint), m.handle, EQD_KS[method]))
+    return m
+end
+
+"K s = [τ2; τ3] for a slip vector [s2; s3] with the model's K s method."
+function ks_product(m::DeviceBP8Model, s::Vector{Float64})
+    length(s) == 2m.nf || error("slip must have $(2m.nf) entries")
+    out = zeros(2m.nf)
+    check(ccall((:eqd_bp8_ks, libeqd), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), m.handle, s, out))
+    return out
+end
+
+"""
+    run_bp8(m; tspan=(0.0, m.par.t_f), reltol=1e-8, saveat=3600.0, maxiters=100000, alg=:tsit5)
+        -> (; t, u, stats)
+
+run_bp8 (src/BP8.jl:442-470): Tsit5 with OrdinaryDiffEq's defaults, stepped on the device;
+alg=:imex runs the implicit-local integrator (L-stable SDIRK4 stages solved by per-node 3 x 3
+Newton blocks) for the stiff Peaceman-well variant.
 """
 function run_bp8(m::DeviceBP8Model; tspan=(0.0, m.par.t_f), reltol=1e-8, saveat=3600.0, verbose=false,
-                 maxiters=100000)
+                 maxiters=100000, alg::Symbol=:tsit5)
+    alg in (:tsit5, :imex) || error("alg must be :tsit5 or :imex, got $alg")
     t0, tf = Float64.(tspan)
     nmax = floor(Int, (tf - t0) / saveat + 1e-9) + 2
     ts = zeros(nmax)
     us = zeros(m.state_len, nmax)     # column per save == the C ABI's row per save
     st = BP8Stats()
-    check(ccall((:eqd_bp8_integrate, libeqd), Cint,
-                (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Int64, Int64, Ptr{Float64}, Ptr{Float64},
-                 Ref{BP8Stats}), m.handle, t0, tf, saveat, reltol, maxiters, nmax, ts, us, st))
+    if alg === :tsit5
+        check(ccall((:eqd_bp8_integrate, libeqd), Cint,
+                    (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Int64, Int64, Ptr{Float64}, Ptr{Float64},
+                     Ref{BP8Stats}), m.handle, t0, tf, saveat, reltol, maxiters, nmax, ts, us, st))
+    else
+        check(ccall((:eqd_bp8_integrate_imex, libeqd), Cint,
+                    (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Int64, Int64, Ptr{Float64}, Ptr{Float64},
+                     Ref{BP8Stats}), m.handle, t0, tf, saveat, reltol, maxiters, nmax, ts, us, st))
+    end
     n = st.nsaved
     return (; t=ts[1:n], u=[us[:, j] for j in 1:n], stats=(; naccept=st.naccept, nreject=st.nreject, nf=st.nrhs),
               retcode=st.retcode == 0 ? :Success : :MaxIters)
@@ -364,7 +396,8 @@ end
 # ---------------------------------------------------------------------------------------
 # Trajectory ensembles (no counterpart in the reference, which runs one trajectory per
 # call): M members of run_bp8 on one model's K, each with its own Q0, a, σ0 (and reltol);
-# K S over all members is one FP64 MFMA GEMM per stage.
+# K S over all members is one FFT convolution per stage for a :toeplitz model, else one FP64
+# MFMA GEMM.
 
 mutable struct DeviceEnsemble
     handle::Ptr{Cvoid}

@@ -252,6 +252,24 @@ eqd_status eqd_bp8_integrate(eqd_bp8* m, double t0, double tf, double saveat, do
 eqd_status eqd_bp8_integrate_imex(eqd_bp8* m, double t0, double tf, double saveat, double reltol,
                                   int64_t maxiters, int64_t nsave_max, double* ts, double* us,
                                   eqd_bp8_stats* stats);
+/* How a model built with stiffness = TOEPLITZ forms K s (src/FaultResponse.jl:378-390 makes each of
+ * K's four blocks a function of node separation, so K s is a 2D correlation with the
+ * 4 (2 n2f - 1)^2-value separation kernel):
+ *   EQD_KS_DIRECT  the correlation summed directly, O(nf^2) per product (inside the stage kernel
+ *                  up to 48 x 48 fault nodes, the default there);
+ *   EQD_KS_FFT     the same correlation as a zero-padded 2D FFT convolution (transform size the
+ *                  power of two >= 2 n2f - 1), O(nf log nf); the default beyond 48 x 48 nodes
+ *                  (fault patches of up to 256 x 256 nodes).
+ * Both use K's own entries; results differ by rounding only. EQD_ERR_ARG for a dense-K model. */
+enum { EQD_KS_DIRECT = 0, EQD_KS_FFT = 1 };
+eqd_status eqd_bp8_set_ks_method(eqd_bp8* m, int method);
+/* K s with the model's current method for a host slip vector s[2nf] = [s2; s3] into
+ * out[2nf] = [tau2; tau3] (host); EQD_ERR_ARG when the method forms K s inside the stage kernel
+ * (EQD_KS_DIRECT at 48 x 48 nodes or fewer). */
+eqd_status eqd_bp8_ks(eqd_bp8* m, const double* s, double* out);
+/* Measurement hook: average device time of one K s (the model's current method) over reps
+ * launches, on the model's initial state. */
+eqd_status eqd_bp8_ks_timing(eqd_bp8* m, int reps, double* ms_per_product);
 /* Device pointer to the saved states of the last integrate: row j (save j) starts at
  * us_dev + j * ld (ld >= state_len, even). */
 eqd_status eqd_bp8_saved_device(const eqd_bp8* m, const double** us_dev, int64_t* ld, int64_t* nsaved);
@@ -274,7 +292,8 @@ eqd_status eqd_bp8_saved_outputs(eqd_bp8* m, int64_t n, const int* idx, int nidx
  * members is one FP64 MFMA GEMM (K resident, S = the members' slip vectors), followed by
  * the per-(node, member) Newton in ln V, aging law and A_p p. A member's trajectory is the
  * one run_bp8 gives for build_model with that member's parameters (no counterpart in the
- * reference, which runs one trajectory per call).
+ * reference, which runs one trajectory per call). For a model built with stiffness = TOEPLITZ
+ * the stage's K S is instead one FFT convolution batched over the members (EQD_KS_FFT).
  * ------------------------------------------------------------------------------ */
 typedef struct eqd_bp8_ensemble eqd_bp8_ensemble;
 
@@ -297,8 +316,9 @@ eqd_status eqd_bp8_ensemble_saved(const eqd_bp8_ensemble* e, int member, double*
  * evaluated in save order as eqd_bp8_saved_vmax does; one GEMM per save over all members.
  * ms: device time (may be NULL). */
 eqd_status eqd_bp8_ensemble_saved_vmax(eqd_bp8_ensemble* e, double* vmax, int64_t n, double* ms);
-/* Measurement hook: average device time of one K S GEMM over all members' current states
- * (reps launches) and its flop count 2 (2nf)^2 members. */
+/* Measurement hook: average device time of one K S over all members' current states (reps
+ * launches) and the flop count 2 (2nf)^2 members of the GEMM. A member ensemble of a TOEPLITZ
+ * model forms K S as one batched FFT convolution (no copy of K; flops reported as 0). */
 eqd_status eqd_bp8_ensemble_gemm_timing(eqd_bp8_ensemble* e, int reps, double* ms_per_gemm, double* flops);
 
 /* ------------------------------------------------------------------------------

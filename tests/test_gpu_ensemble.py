@@ -1,4 +1,5 @@
-"""Trajectory ensembles (members in lockstep stages, K S as one FP64 MFMA GEMM per stage)
+"""Trajectory ensembles (members in lockstep stages, K S as one FP64 MFMA GEMM per stage for a
+dense K, as one batched FFT convolution for a :toeplitz model)
 through the C ABI, member by member against the CPU oracle run with that member's Q0, a
 and sigma0.
 
@@ -138,9 +139,9 @@ def test_bad_arguments(toeplitz_K):
 
 def test_production_scale_ensemble_matches_single_models():
     """The dz = 20 m ensemble configuration (nf = 6561, K 13122^2 from the device Toeplitz build,
-    K S as the 13122 x 13122 x M MFMA GEMM): member 0 (base parameters) against run_bp8 of the
-    same model (K s from the separation kernel), and saved V_max of two perturbed members against
-    evaluate! of one-member dense-K models with their parameters."""
+    K S as one 256 x 256 FFT convolution batched over the members): member 0 (base parameters)
+    against run_bp8 of the same model (its own FFT K s), and saved V_max of two perturbed members
+    against evaluate! of one-member dense-K models with their parameters."""
     print("[ensemble dz=20 nf=6561 M=16 3 h]", flush=True)
     par = bp8.benchmark_parameters(l_f=800.0, dz=20.0)
     m = bp8.build_model(par=par, dz=20.0, L_fault=800.0, L_normal=400.0, stiffness="toeplitz", precision="mixed")
