@@ -424,13 +424,90 @@ def bench_ensemble(args):
     return res
 
 
+def _serve_stream(sv, gids, axes, specs, weights, rate, seconds, seed):
+    """One seeded Poisson stream (mean `rate` req/s for `seconds` s) through the server; returns
+    latency percentiles, throughput and the backlog the stream leaves when its last request
+    arrives."""
+    rng = np.random.default_rng(seed)
+    n_req = rng.poisson(rate * seconds)
+    arrivals = np.sort(rng.uniform(0.0, seconds, n_req))
+    reqs = []
+    for a in arrivals:
+        gi = int(rng.choice(len(specs), p=weights))
+        nf = sv.grids[gi]["nf"]
+        if rng.random() < 0.5:
+            k = int(rng.integers(1, 65))
+            reqs.append((a, gi, "columns", rng.choice(2 * nf, size=k, replace=False), None))
+        else:
+            x = axes[gi]
+            X, Y = np.meshgrid(x, x, indexing="ij")
+            w = rng.uniform(100.0, 400.0)
+            c2, c3 = rng.uniform(x[0], x[-1], 2)
+            bump = (1e-3 * np.exp(-((X - c2) ** 2 + (Y - c3) ** 2) / (2 * w * w))).ravel(order="F")
+            zero = np.zeros(nf)
+            reqs.append((a, gi, "forward", None, (bump, zero) if rng.random() < 0.5 else (zero, bump)))
+    pending = {}
+    lat = [[] for _ in specs]
+    cols_done = 0
+    cols_offered = sum(1 if r[3] is None else len(r[3]) for r in reqs)
+    backlog_at_last_arrival = None
+    t_last_done = 0.0
+    i = 0
+    start = time.perf_counter()
+    t_report = 10.0
+    while i < len(reqs) or pending:
+        if time.perf_counter() - start > t_report:
+            print(f"bench: serving {rate} req/s: {i} submitted, {i - len(pending)} done at {t_report:.0f} s",
+                  file=sys.stderr, flush=True)
+            t_report += 10.0
+        now = time.perf_counter() - start
+        while i < len(reqs) and reqs[i][0] <= now:
+            a, gi, kind, cols, slip = reqs[i]
+            rid = sv.submit_columns(gids[gi], cols) if kind == "columns" else sv.submit_forward(gids[gi], *slip)
+            pending[rid] = (a, gi, 1 if cols is None else len(cols))
+            i += 1
+            if i == len(reqs):
+                backlog_at_last_arrival = dict(requests=len(pending), columns=int(sum(v[2] for v in pending.values())))
+        busy, queued = sv.step()
+        for rid in sv.completed():
+            a, gi, nc = pending.pop(rid)
+            sv.result(rid, gids[gi], nc)
+            t_last_done = time.perf_counter() - start
+            lat[gi].append(t_last_done - a)
+            cols_done += nc
+        if busy == 0 and queued == 0 and i < len(reqs):
+            time.sleep(max(0.0, min(reqs[i][0] - (time.perf_counter() - start), 0.002)))
+        if time.perf_counter() - start > seconds * 40 + 180:
+            raise RuntimeError("serving run did not drain")
+    wall = time.perf_counter() - start
+    allat = np.concatenate([np.array(l) for l in lat if l]) * 1e3
+    info = dict(offered_req_per_s=rate, duration_s=seconds, requests=len(reqs), offered_columns=cols_offered,
+                wall_s=round(wall, 2), drain_s=round(wall - (arrivals[-1] if len(arrivals) else 0.0), 2),
+                completed_req_per_s=round(len(reqs) / wall, 2), columns=cols_done,
+                columns_per_s=round(cols_done / wall, 1), backlog_at_last_arrival=backlog_at_last_arrival,
+                p50_ms=round(float(np.percentile(allat, 50)), 1), p99_ms=round(float(np.percentile(allat, 99)), 1),
+                mean_ms=round(float(allat.mean()), 1))
+    info["by_grid"] = {f"dz{int(d)}_Lf{int(f)}_Ln{int(n)}": (
+        dict(requests=len(l), p50_ms=round(float(np.percentile(np.array(l) * 1e3, 50)), 1),
+             p99_ms=round(float(np.percentile(np.array(l) * 1e3, 99)), 1)) if l else dict(requests=0))
+        for (d, f, n), l in zip(specs, lat)}
+    print(f"bench: serving {rate} req/s: {len(reqs)} requests in {wall:.1f} s (drain {info['drain_s']} s): "
+          f"p50 {info['p50_ms']} ms, p99 {info['p99_ms']} ms, {info['columns_per_s']} columns/s, "
+          f"{info['completed_req_per_s']} req/s completed", file=sys.stderr, flush=True)
+    return info
+
+
 def bench_serving(args):
-    """Serving loop under a seeded Poisson stream (mean args.serve_rate req/s for
-    args.serve_seconds s) over three grids with l_f = L_fault: (dz, L_fault, L_normal) =
-    (100, 800, 800) -- the legal form of (100, 800, 400), whose n1 = 5 < 9 -- (50, 800, 400) and
-    (25, 1600, 800), weights 0.5 / 0.35 / 0.15. A request is, with equal probability, k ~ U{1..64}
-    unit-slip columns at random nodes and directions, or a forward solve with a Gaussian slip bump
-    (amplitude 1e-3 m, width U[100, 400] m, centre uniform on Omega_f, random direction)."""
+    """Serving loop under seeded Poisson streams over three grids with l_f = L_fault: (dz,
+    L_fault, L_normal) = (100, 800, 800) -- the legal form of (100, 800, 400), whose n1 = 5 < 9 --
+    (50, 800, 400) and (25, 1600, 800), weights 0.5 / 0.35 / 0.15. A request is, with equal
+    probability, k ~ U{1..64} unit-slip columns at random nodes and directions, or a forward solve
+    with a Gaussian slip bump (amplitude 1e-3 m, width U[100, 400] m, centre uniform on Omega_f,
+    random direction). Two streams on the same server: args.serve_rate req/s (default 10) for
+    args.serve_seconds s, and args.serve_rate_high req/s (default 50, the offered load of the serving
+    configuration) for args.serve_seconds_high s. The second exceeds what one device serves on this
+    mix; its completed req/s and columns/s are the saturation throughput, and its backlog when the
+    last request arrives and its drain time say by how much."""
     from eqd.serve import Server
     mu, nu = 32.04e9, 0.25
     lam = 2 * mu * nu / (1 - 2 * nu)
@@ -445,61 +522,12 @@ def bench_serving(args):
         n2f = sv.grids[-1]["n2f"]
         axes.append(np.linspace(-Lf, Lf, n2f))
     t_setup = time.perf_counter() - t0
-    rng = np.random.default_rng(20261203)
-    n_req = rng.poisson(args.serve_rate * args.serve_seconds)
-    arrivals = np.sort(rng.uniform(0.0, args.serve_seconds, n_req))
-    reqs = []
-    for a in arrivals:
-        gi = int(rng.choice(3, p=weights))
-        nf = sv.grids[gi]["nf"]
-        if rng.random() < 0.5:
-            k = int(rng.integers(1, 65))
-            reqs.append((a, gi, "columns", rng.choice(2 * nf, size=k, replace=False), None))
-        else:
-            x = axes[gi]
-            X, Y = np.meshgrid(x, x, indexing="ij")
-            w = rng.uniform(100.0, 400.0)
-            c2, c3 = rng.uniform(x[0], x[-1], 2)
-            bump = (1e-3 * np.exp(-((X - c2) ** 2 + (Y - c3) ** 2) / (2 * w * w))).ravel(order="F")
-            zero = np.zeros(nf)
-            reqs.append((a, gi, "forward", None, (bump, zero) if rng.random() < 0.5 else (zero, bump)))
-    info = {}
-    pending = {}
-    lat = [[] for _ in specs]
-    cols_done = 0
-    i = 0
-    start = time.perf_counter()
-    while i < len(reqs) or pending:
-        now = time.perf_counter() - start
-        while i < len(reqs) and reqs[i][0] <= now:
-            a, gi, kind, cols, slip = reqs[i]
-            rid = sv.submit_columns(gids[gi], cols) if kind == "columns" else sv.submit_forward(gids[gi], *slip)
-            pending[rid] = (a, gi, 1 if cols is None else len(cols))
-            i += 1
-        busy, queued = sv.step()
-        for rid in sv.completed():
-            a, gi, nc = pending.pop(rid)
-            sv.result(rid, gids[gi], nc)
-            lat[gi].append(time.perf_counter() - start - a)
-            cols_done += nc
-        if busy == 0 and queued == 0 and i < len(reqs):
-            time.sleep(max(0.0, min(reqs[i][0] - (time.perf_counter() - start), 0.002)))
-        if time.perf_counter() - start > args.serve_seconds * 20 + 120:
-            raise RuntimeError("serving run did not drain")
-    wall = time.perf_counter() - start
-    allat = np.concatenate([np.array(l) for l in lat if l]) * 1e3
-    info.update(offered_req_per_s=args.serve_rate, duration_s=args.serve_seconds, requests=len(reqs),
-                wall_s=round(wall, 2), columns=cols_done, columns_per_s=round(cols_done / wall, 1),
-                p50_ms=round(float(np.percentile(allat, 50)), 1), p99_ms=round(float(np.percentile(allat, 99)), 1),
-                mean_ms=round(float(allat.mean()), 1), setup_s=round(t_setup, 2),
-                hbm_pool_GB=round(sum(g["hbm_bytes"] for g in sv.grids) / 1e9, 2),
+    info = _serve_stream(sv, gids, axes, specs, weights, args.serve_rate, args.serve_seconds, 20261203)
+    info.update(setup_s=round(t_setup, 2), hbm_pool_GB=round(sum(g["hbm_bytes"] for g in sv.grids) / 1e9, 2),
                 lanes=[g["slots"] for g in sv.grids])
-    info["by_grid"] = {f"dz{int(d)}_Lf{int(f)}_Ln{int(n)}": (
-        dict(requests=len(l), p50_ms=round(float(np.percentile(np.array(l) * 1e3, 50)), 1),
-             p99_ms=round(float(np.percentile(np.array(l) * 1e3, 99)), 1)) if l else dict(requests=0))
-        for (d, f, n), l in zip(specs, lat)}
-    print(f"bench: serving {len(reqs)} requests in {wall:.1f} s: p50 {info['p50_ms']} ms, p99 {info['p99_ms']} ms, "
-          f"{info['columns_per_s']} columns/s", file=sys.stderr, flush=True)
+    if args.serve_rate_high > 0 and args.serve_seconds_high > 0:
+        info["high_rate"] = _serve_stream(sv, gids, axes, specs, weights, args.serve_rate_high,
+                                          args.serve_seconds_high, 20261204)
     return info
 
 
@@ -540,8 +568,11 @@ def main():
     ap.add_argument("--ens_pw_maxiters", type=int, default=3000,
                     help="step budget per Peaceman-well member (a longer run returns MaxIters)")
     ap.add_argument("--serve_seconds", type=float, default=10.0)
-    ap.add_argument("--serve_rate", type=float, default=10.0,
-                    help="mean request rate; 50 req/s over this grid mix exceeds one GPU (see DESIGN.md)")
+    ap.add_argument("--serve_rate", type=float, default=10.0, help="mean request rate of the first stream")
+    ap.add_argument("--serve_rate_high", type=float, default=50.0,
+                    help="mean request rate of the second stream (0: skip); 50 req/s over this grid mix exceeds "
+                         "one GPU, so the stream measures the saturation throughput and the backlog")
+    ap.add_argument("--serve_seconds_high", type=float, default=5.0)
     ap.add_argument("--serve_lanes", type=int, default=512, help="lanes per grid in the serving pools")
     ap.add_argument("--serve_lanes_dz25", type=int, default=64,
                     help="lanes of the dz = 25 m pool: one tile already fills the device there, so a larger "
