@@ -272,4 +272,24 @@ def effective_stress_report(m: BP8Model) -> dict:
                 nodes=len(idx), fraction=len(idx) / m.nf, radius=radius)
 
 
+def process_zone(par: dict, sigma_bar: float | None = None) -> float:
+    """process_zone(par, sigma_bar) = mu D_RS / (b sigma_bar) (src/BP8.jl:297-305): the rate-and-state
+    nucleation length, at the initial sigma0 by default."""
+    sb = par["sigma0"] if sigma_bar is None else sigma_bar
+    return par["mu"] * par["D_RS"] / (par["b"] * sb)
+
+
+def resolution_report(m: BP8Model) -> dict:
+    """resolution_report(m) (src/BP8.jl:307-326): cells of dz per process zone L_b (at sigma0, and
+    at the lowest effective stress seen, floored at sigma_min) and per Gaussian source width;
+    converged when L_b / dz >= 3."""
+    par = m.par
+    dz = m.grid_info.get("dz", par["dz"])
+    lb0 = process_zone(par)
+    lo = effective_stress_report(m)["sigma_lowest"]
+    sb_lo = max(lo, par["sigma_min"]) if math.isfinite(lo) else par["sigma0"]
+    return dict(dz=dz, L_b=lb0, cells_per_Lb=lb0 / dz, L_b_at_lowest_sigma=process_zone(par, sb_lo),
+                cells_per_L_gauss=par["L_gauss"] / dz, converged=lb0 / dz >= 3)
+
+
 from .outputs import saved_outputs, station_locations, write_outputs  # noqa: E402  (section 4 output files)
