@@ -691,6 +691,10 @@ def main():
                                      "sample_columns": len(sample)}
         out["batched_fp64"] = {"value": round(ncols / t64, 2), "unit": "K-columns/s"}
         out["speedup_vs_plain_fp64"] = round(value / plain_value, 1)
+        # the ratio's two factors: batching (64-column tiles, device-side alpha/beta, graphs) in
+        # FP64, then FP32 CG inside FP64 reliable updates
+        out["speedup_decomposition"] = {"batched_fp64_vs_plain_fp64": round((ncols / t64) / plain_value, 1),
+                                        "mixed_vs_batched_fp64": round(value / (ncols / t64), 2)}
         out["K_rel_err_vs_fp64"] = err_vs_fp64
 
         if not args.no_cpu:
