@@ -668,11 +668,17 @@ def main():
         # whole-iteration metric of the survey: 10 vector passes x DOF x 4 B per column-iteration
         col_iters = rep["iterations"] / max(1, rep["solves"]) * ncols
         survey_gbs = 10 * dof * 4 * col_iters / (ms_per_step / 1e3)
-        # full-line pencils (one 16-node segment covers x1: 2 n1 - 3 <= 16)
-        full = "true" if 2 * g.n1 - 3 <= 16 else "false"
+        # x1 lines of 9 to 16 nodes: the x1-inner layout and its x3-marching stencil; longer lines:
+        # full-line (one 16-node segment) or segmented pencils
+        nx = 2 * g.n1 - 3
+        full = "true" if nx <= 16 else "false"
         tk = eqd.solver_report(fet.rs)["tile_k"]
-        w15 = ",15" if 2 * g.n1 - 3 <= 15 else ""  # x1 lines of at most 15 nodes use the 15-wide pencil
-        stencil_name = (f"k_pencil<float,float,0,{full}{w15}>" if tk == 64 else f"k_apply_dot<float,{tk}>")
+        if tk != 64:
+            stencil_name = f"k_apply_dot<float,{tk}>"
+        elif 9 <= nx <= 16:
+            stencil_name = "k_xi_pencil<float,float,0,3>"
+        else:
+            stencil_name = f"k_pencil<float,float,0,{full}>"
         out["roofline"] = {"bound": "hbm", "kernel": stencil_name,
                            "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK, 4), "traffic": measured_traffic(stencil_name),

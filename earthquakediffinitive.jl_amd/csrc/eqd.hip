@@ -40,7 +40,10 @@ void validate_grid(const eqd_grid_desc& g) {
     // byte offsets: the largest one (FP64, 3 components per stored node) must stay below 2^31 B.
     const int64_t NN = (int64_t)(2 * g.n1 - 3) * (g.n23 - 2) * (g.n23 - 2);
     const int64_t tile_bytes = NN * 3 * 64 * (int64_t)sizeof(double);
-    if (tile_bytes > (int64_t)INT32_MAX)
+    // x1-inner tiles (x1 lines of 9 to 16 nodes) pad the lines to 16 nodes
+    const int64_t NXm = 2 * g.n1 - 3;
+    const int64_t xi_bytes = (NXm >= 9 && NXm <= 16) ? (int64_t)(g.n23 - 2) * (g.n23 - 2) * 3 * 1024 * 8 : 0;
+    if (tile_bytes > (int64_t)INT32_MAX || xi_bytes > (int64_t)INT32_MAX)
         throw Error(EQD_ERR_GRID, "grid too large: " + std::to_string(NN) + " stored nodes make a 64-column FP64 tile of " +
                                       std::to_string(tile_bytes) + " B, above the 2^31 B a tile vector may span (at most " +
                                       std::to_string((int64_t)INT32_MAX / (3 * 64 * 8)) + " stored nodes)");
@@ -121,6 +124,9 @@ struct eqd_op {
     // segments with x1 halos (p64's bands are padded to 16 rows, the pencil reads 15)
     PencilView<double> p64w{};
     bool p64w_ok = false;
+    // x1 lines of at most 16 nodes: the 64-column tiles of the mixed-precision engine use the
+    // x1-inner layout and its kernels (xinner.h)
+    bool xi = false;
     int G = kGroups;  // workgroups per tile of the node-parallel kernels (set at creation)
     int nfold = 0;    // > 0: block totals of the two-level partial sum (G > kFoldAbove)
     int fchunk = 0;   // groups per first-level block
@@ -310,7 +316,7 @@ void eqd::host::solver_fault_dims(const eqd_solver* s, int& nf, int& n2f, int& n
 namespace {
 
 size_t bytes_per_column(const eqd_op& op, int prec) {
-    const size_t n = (size_t)op.NN * 3;
+    const size_t n = (prec == EQD_PREC_MIXED && op.xi) ? xi_tile_elems(op.tb.NY) / 64 : (size_t)op.NN * 3;
     const size_t t = prec == EQD_PREC_MIXED ? 4 : 8;
     return n * (8 + 8 + 3 * t + (prec == EQD_PREC_MIXED ? 4 : 0)) + 2 * (size_t)op.g.n23 * op.g.n23 * 8;
 }
@@ -325,7 +331,8 @@ void ensure_ws(eqd_solver* s, int K, int ntiles, int prec) {
     w->K = K;
     w->ntiles = ntiles;
     w->prec = prec;
-    w->nvec = (size_t)ntiles * op.NN * 3 * K;
+    w->nvec = (K == 64 && prec == EQD_PREC_MIXED && op.xi) ? (size_t)ntiles * xi_tile_elems(op.tb.NY)
+                                                           : (size_t)ntiles * op.NN * 3 * K;
     const size_t slots = (size_t)ntiles * K;
     const size_t t = prec == EQD_PREC_MIXED ? sizeof(float) : sizeof(double);
     w->b.alloc(w->nvec * 8);
@@ -347,6 +354,23 @@ void ensure_ws(eqd_solver* s, int K, int ntiles, int prec) {
 
 template <typename T, int K>
 struct Engine {
+    // The workspace holds 64-column mixed-precision tiles of an x1-inner grid (xinner.h). Depends
+    // on the workspace, not on T: extraction runs as Engine<double, K> for both precisions.
+    static bool xi(const eqd_solver* s) {
+        return K == 64 && s->op->xi && s->ws && s->ws->prec == EQD_PREC_MIXED;
+    }
+
+    // x += xc, xc = 0 for every column (mask == nullptr) or exactly the masked ones.
+    static void xaccum_all(eqd_solver* s, hipStream_t st, dim3 grid, const int* mask) {
+        Workspace& w = *s->ws;
+        if (xi(s))
+            hipLaunchKernelGGL(k_xi_xaccum, grid, dim3(kBlock), 0, st, s->op->tb.NX, s->op->tb.NY, w.x.as<double>(),
+                               w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1, mask);
+        else
+            hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, s->op->NN, w.x.as<double>(),
+                               w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1, mask);
+    }
+
     // One CG iteration for every tile in flight (device-side alpha/beta). With a
     // timer, every launch is bracketed by events (measurement runs, no graph).
     static void iteration(eqd_solver* s, hipStream_t st, KTimer* kt = nullptr) {
@@ -391,6 +415,14 @@ struct Engine {
                 }
                 // x1 lines of at most 15 nodes (n1 = 9: the c1, c2 grids) take a 15-wide pencil, so
                 // that no neighbour-line load falls past the line's end (stencil 559 -> 535 us)
+                if constexpr (std::is_same<T, float>::value) {
+                    if (xi(s)) {
+                        hipLaunchKernelGGL((k_xi_pencil<float, float, 0, EQD_XI_Z>), grid, dim3(kBlock), 0, st, op.p32,
+                                           p, q, (const double*)nullptr, (float*)nullptr, cs, on, part,
+                                           (const double*)nullptr);
+                        return;
+                    }
+                }
                 if (pv.nseg == 1 && std::is_same<T, float>::value && pv.NX <= 15)
                     hipLaunchKernelGGL((k_pencil<T, T, 0, true, 15>), grid, dim3(kBlock), pencil_lds_bytes(pv), st,
                                        pv, op.NN, p, q, (const T*)nullptr, (T*)nullptr, cs, on, part,
@@ -420,6 +452,17 @@ struct Engine {
             T* xu = nullptr;
             if constexpr (std::is_same<T, float>::value) xu = w.xc.as<float>();
             else xu = w.x.as<double>();
+            if constexpr (K == 64 && std::is_same<T, float>::value) {
+                if (xi(s)) {
+                    if (dinv)
+                        hipLaunchKernelGGL((k_xi_update<true>), grid, dim3(kBlock), 0, st, op.tb.NX, op.tb.NY,
+                                           op.tb.mface, r, q, cs, on, part, (const float*)dinv);
+                    else
+                        hipLaunchKernelGGL((k_xi_update<false>), grid, dim3(kBlock), 0, st, op.tb.NX, op.tb.NY,
+                                           op.tb.mface, r, q, cs, on, part, (const float*)nullptr);
+                    return;
+                }
+            }
             if constexpr (K == 64) {
                 if (dinv)
                     hipLaunchKernelGGL((k_update64<T, true>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface,
@@ -441,6 +484,17 @@ struct Engine {
             T* xu = nullptr;
             if constexpr (std::is_same<T, float>::value) xu = w.xc.as<float>();
             else xu = w.x.as<double>();
+            if constexpr (K == 64 && std::is_same<T, float>::value) {
+                if (xi(s)) {
+                    if (dinv)
+                        hipLaunchKernelGGL((k_xi_pupdate<true>), grid, dim3(kBlock), 0, st, op.tb.NX, op.tb.NY, xu, p,
+                                           r, cs, (const float*)dinv);
+                    else
+                        hipLaunchKernelGGL((k_xi_pupdate<false>), grid, dim3(kBlock), 0, st, op.tb.NX, op.tb.NY, xu,
+                                           p, r, cs, (const float*)nullptr);
+                    return;
+                }
+            }
             if constexpr (K == 64) {
                 if (dinv)
                     hipLaunchKernelGGL((k_pupdate64<T, true>), grid, dim3(kBlock), 0, st, op.NN, xu, p, r, cs, dinv);
@@ -452,15 +506,22 @@ struct Engine {
         });
         if (mixed) {
             timed(4, [&] {
-                hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, op.NN, w.x.as<double>(), w.xc.as<float>(),
-                                   cs, rel, 0, (const int*)nullptr);
+                if (xi(s))
+                    hipLaunchKernelGGL(k_xi_xaccum, grid, dim3(kBlock), 0, st, op.tb.NX, op.tb.NY, w.x.as<double>(),
+                                       w.xc.as<float>(), cs, rel, 0, (const int*)nullptr);
+                else
+                    hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, op.NN, w.x.as<double>(),
+                                       w.xc.as<float>(), cs, rel, 0, (const int*)nullptr);
             });
             timed(5, [&] {
                 if constexpr (K == 64)
                 {
                     const double* x64 = w.x.as<double>();
                     const double* b64 = w.b.as<double>();
-                    if (op.p64.nseg == 1)
+                    if (xi(s))
+                        hipLaunchKernelGGL((k_xi_pencil<double, T, 1, 1>), grid, dim3(kBlock), 0, st, op.p64, x64,
+                                           (double*)nullptr, b64, r, cs, rel, part, dinv64);
+                    else if (op.p64.nseg == 1)
                         hipLaunchKernelGGL((k_pencil<double, T, 1, true>), grid, dim3(kBlock),
                                            pencil_lds_bytes(op.p64), st, op.p64, op.NN, x64, (double*)nullptr, b64, r,
                                            cs, rel, part, dinv64);
@@ -491,9 +552,14 @@ struct Engine {
         Workspace& w = *s->ws;
         const eqd_op& op = *s->op;
         dim3 grid(s->op->G, w.ntiles);
-        hipLaunchKernelGGL((k_selfdot<double, K>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface,
-                           w.b.as<double>(), w.partial.as<double>(),
-                           s->o.precond == EQD_PRECOND_JACOBI ? op.dinv64 : nullptr);
+        if (xi(s))
+            hipLaunchKernelGGL(k_xi_selfdot, grid, dim3(kBlock), 0, st, op.tb.NX, op.tb.NY, op.tb.mface,
+                               w.b.as<double>(), w.partial.as<double>(),
+                               s->o.precond == EQD_PRECOND_JACOBI ? op.dinv64 : nullptr);
+        else
+            hipLaunchKernelGGL((k_selfdot<double, K>), grid, dim3(kBlock), 0, st, op.NN, op.tb.NX, op.tb.mface,
+                               w.b.as<double>(), w.partial.as<double>(),
+                               s->o.precond == EQD_PRECOND_JACOBI ? op.dinv64 : nullptr);
         const int64_t itmax = s->o.itmax > 0 ? s->o.itmax : 2 * op.ntot;
         if (op.nfold)
             hipLaunchKernelGGL((k_fold_partials<K>), dim3(op.nfold, w.ntiles), dim3(K * 16), 0, st, op.G, op.fchunk,
@@ -537,8 +603,7 @@ struct Engine {
         }
         if (std::is_same<T, float>::value) {
             dim3 grid(s->op->G, w.ntiles);
-            hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, s->op->NN, w.x.as<double>(),
-                               w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1, (const int*)nullptr);
+            xaccum_all(s, st, grid, nullptr);
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipStreamSynchronize(st));
@@ -615,8 +680,7 @@ struct Engine {
         }
         if (std::is_same<T, float>::value) {  // fold any correction still held in FP32
             dim3 grid(s->op->G, w.ntiles);
-            hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, s->op->NN, w.x.as<double>(),
-                               w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1, (const int*)nullptr);
+            xaccum_all(s, st, grid, nullptr);
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(e1, st));
@@ -652,8 +716,7 @@ struct Engine {
         }
         if (std::is_same<T, float>::value) {  // fold any correction still held in FP32
             dim3 grid(s->op->G, w.ntiles);
-            hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, s->op->NN, w.x.as<double>(),
-                               w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1, (const int*)nullptr);
+            xaccum_all(s, st, grid, nullptr);
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipStreamSynchronize(st));
@@ -739,10 +802,15 @@ struct Engine {
         Workspace& w = *s->ws;
         const eqd_op& op = *s->op;
         dim3 grid(s->op->G, w.ntiles);
-        hipLaunchKernelGGL((k_rhs<T, K>), grid, dim3(kBlock), 0, s->stream, op.rv, op.NN, w.fields.as<double>(),
-                           w.ntiles * K, w.b.as<double>(), w.x.as<double>(), w.r.as<T>(), w.p.as<T>(),
-                           std::is_same<T, float>::value ? w.xc.as<float>() : nullptr, mask,
-                           s->o.precond == EQD_PRECOND_JACOBI ? op.dinv64 : nullptr);
+        if (xi(s))
+            hipLaunchKernelGGL(k_xi_rhs, grid, dim3(kBlock), 0, s->stream, op.rv, w.fields.as<double>(), w.ntiles * K,
+                               w.b.as<double>(), w.x.as<double>(), w.r.as<float>(), w.p.as<float>(),
+                               w.xc.as<float>(), mask, s->o.precond == EQD_PRECOND_JACOBI ? op.dinv64 : nullptr);
+        else
+            hipLaunchKernelGGL((k_rhs<T, K>), grid, dim3(kBlock), 0, s->stream, op.rv, op.NN, w.fields.as<double>(),
+                               w.ntiles * K, w.b.as<double>(), w.x.as<double>(), w.r.as<T>(), w.p.as<T>(),
+                               std::is_same<T, float>::value ? w.xc.as<float>() : nullptr, mask,
+                               s->o.precond == EQD_PRECOND_JACOBI ? op.dinv64 : nullptr);
         HIPCHK(hipGetLastError());
     }
 
@@ -751,8 +819,7 @@ struct Engine {
         if constexpr (std::is_same<T, float>::value) {
             Workspace& w = *s->ws;
             dim3 grid(s->op->G, w.ntiles);
-            hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, s->stream, s->op->NN, w.x.as<double>(),
-                               w.xc.as<float>(), w.cs.as<ColState>(), w.tile_on.as<int>(), 1, mask);
+            xaccum_all(s, s->stream, grid, mask);
             HIPCHK(hipGetLastError());
         }
     }
@@ -762,8 +829,12 @@ struct Engine {
         const eqd_op& op = *s->op;
         const int npb = kBlock / K;
         dim3 grid((op.nf + npb - 1) / npb, w.ntiles);
-        hipLaunchKernelGGL((k_extract<K>), grid, dim3(kBlock), 0, s->stream, op.ev, op.NN, w.x.as<double>(),
-                           w.fields.as<double>(), w.outpos.as<int64_t>(), Kdev, ldK, (int64_t)0, mask);
+        if (xi(s))
+            hipLaunchKernelGGL((k_extract<K, true>), grid, dim3(kBlock), 0, s->stream, op.ev, op.NN, w.x.as<double>(),
+                               w.fields.as<double>(), w.outpos.as<int64_t>(), Kdev, ldK, (int64_t)0, mask);
+        else
+            hipLaunchKernelGGL((k_extract<K>), grid, dim3(kBlock), 0, s->stream, op.ev, op.NN, w.x.as<double>(),
+                               w.fields.as<double>(), w.outpos.as<int64_t>(), Kdev, ldK, (int64_t)0, mask);
         HIPCHK(hipGetLastError());
     }
 };
@@ -1268,6 +1339,11 @@ eqd_status eqd_operator_create(const eqd_grid_desc* g, eqd_op** out) {
         op->v32 = op->view<float>();
         op->p64 = op->pview<double>();
         op->p32 = op->pview<float>();
+        {
+            const char* sv = std::getenv("EQD_STENCIL");
+            // 9 <= NX keeps the FP64 bands at 16 rows (two 8-node segments), as the kernels read them
+            op->xi = op->tb.NX >= 9 && op->tb.NX <= 16 && !(sv && std::strcmp(sv, "pencil") == 0);
+        }
         if (op->tb.NX > pencil_s<double>() && op->tb.NX <= 15) {
             op->p64w = op->p64;
             op->p64w.nseg = 1;

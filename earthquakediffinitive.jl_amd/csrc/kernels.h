@@ -830,7 +830,8 @@ __global__ __launch_bounds__(kBlock) void k_selfdot(int NN, int NX, int mface, c
 // Traction change on Omega_f from the merged solution u and the slip fields:
 // dtau_c = 1/2 (T_c U- + T_c U+), U = P u + chi     (src/FaultResponse.jl:201-209)
 // Writes K[row, col] with row = (c-1)*nf + a + n2f*b (column-major, leading dim ldK).
-template <int K>
+// XI: u in the x1-inner layout of 64-column tiles (see xinner.h).
+template <int K, bool XI = false>
 __global__ void k_extract(ExtractView ev, int NN, const double* __restrict__ u, const double* __restrict__ fields,
                           const int64_t* __restrict__ colid, double* __restrict__ Kout, int64_t ldK,
                           int64_t col_offset, const int* __restrict__ mask) {
@@ -845,8 +846,13 @@ __global__ void k_extract(ExtractView ev, int NN, const double* __restrict__ u, 
     const int iy = ev.i2lo + k % ev.n2f, iz = ev.i2lo + k / ev.n2f;
     const bool yin = iy >= 1 && iy <= n23 - 2, zin = iz >= 1 && iz <= n23 - 2;
     const double* f = fields + ((size_t)tile * K + lane) * 2 * n23 * n23;
-    const double* ut = u + (size_t)tile * NN * 3 * K;
-    auto ld = [&](int m, int y, int z, int c) { return ut[((size_t)(m + NX * (y + NY * z)) * 3 + c) * K + lane]; };
+    const double* ut = u + (XI ? (size_t)NY * NY * 3 * 1024 : (size_t)NN * 3 * K) * tile;
+    auto ld = [&](int m, int y, int z, int c) {
+        if constexpr (XI)
+            return ut[((size_t)((y + NY * z) * 3 + c) << 10) + ((lane >> 2) << 6) + (m << 2) + (lane & 3)];
+        else
+            return ut[((size_t)(m + NX * (y + NY * z)) * 3 + c) * K + lane];
+    };
     double out[2];
     for (int d = 0; d < 2; ++d) {
         const int c = d + 1;
@@ -871,6 +877,7 @@ __global__ void k_extract(ExtractView ev, int NN, const double* __restrict__ u, 
 }  // namespace eqd
 
 #include "pencil.h"
+#include "xinner.h"
 
 namespace eqd {
 

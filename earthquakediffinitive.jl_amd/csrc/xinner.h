@@ -1,0 +1,646 @@
+// x1-inner tile layout and its kernels (64-column tiles of the mixed-precision engine on grids
+// whose merged x1 lines have at most 16 nodes: n1 <= 9, the dz = 50 m and the dz = 100 m,
+// L_normal = 800 m grids). Included by kernels.h after pencil.h.
+//
+// Layout. A tile's vector holds, for every x2-x3 line (y, z) and component c, one block of
+// 16 column quads x 16 x1 nodes x 4 columns:
+//     element (m, y, z, c, column j) = ((y + NY z) 3 + c) 1024 + (j / 4) 64 + 4 m + j % 4,
+// x1 nodes m >= NX being zero padding. A 16-byte access holds four columns of one node, so
+// 16 lanes with consecutive nodes read 256 contiguous bytes, and a wave's load fetches a whole
+// (line, component) for 16 columns. The FP64 vectors b and x use the same element order.
+//
+// Stencil. Lanes are (x1 node n = lane % 16, column quad lane / 16); wave w of a block takes the
+// columns 16 w .. 16 w + 15, so one 16-byte load per lane (32 bytes in FP64) fetches the line
+// for the wave's columns. The x1 operators act across the 16 lanes of a row through DPP row
+// shifts, each lane holding the band row of its own node. A block marches the x3 planes of a
+// unit of Z pencils (x2 row y, planes z0 .. z0 + Z - 1): a line loaded once serves every pencil
+// of the unit that needs it, and the mixed x2-x3 term is sum-factorised per plane.
+//
+// Vector kernels. A thread owns one 16-byte slot per visited block: node m = t % 16 and column
+// quad t / 16 are the same for every slot it visits (the grid strides are multiples of 256
+// slots), so the face weight, the padding test and the per-column sums are per thread.
+#pragma once
+#include <utility>
+
+#ifndef EQD_XI_Z
+#define EQD_XI_Z 3  // pencils per unit of the FP32 stencil
+#endif
+#ifndef EQD_XI_WAVES
+#define EQD_XI_WAVES 3
+#endif
+
+namespace eqd {
+
+constexpr int kXiBlock = 1024;  // elements of one (line, component) block
+
+// Elements per tile vector of an x1-inner grid.
+__host__ __device__ inline size_t xi_tile_elems(int NY) { return (size_t)NY * NY * 3 * kXiBlock; }
+
+__device__ __forceinline__ size_t xi_elem(int yz, int c, int m, int j) {
+    return ((size_t)(yz * 3 + c) << 10) + ((j >> 2) << 6) + (m << 2) + (j & 3);
+}
+
+template <typename T>
+using xv4 = T __attribute__((ext_vector_type(4)));
+
+// Four columns of one node through a buffer resource: 16 bytes (FP32) or 32 bytes (FP64).
+template <typename T>
+__device__ __forceinline__ xv4<T> xload(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(xv4<T>, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+    } else {
+        typedef unsigned int u8v __attribute__((ext_vector_type(8)));
+        const auto lo = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+        const auto hi = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16, soff, 0);
+        const u8v w = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        return __builtin_bit_cast(xv4<T>, w);
+    }
+}
+
+// Value of lane n + D of the same 16-lane row (0 past the row's ends): DPP row_shl / row_shr.
+template <int D>
+__device__ __forceinline__ int xnb32(int v) {
+    constexpr int ctrl = D > 0 ? (0x100 + D) : (0x110 - D);
+    return __builtin_amdgcn_update_dpp(0, v, ctrl, 0xF, 0xF, true);
+}
+template <int D, typename T>
+__device__ __forceinline__ T xnb(T v) {
+    if constexpr (D == 0) {
+        return v;
+    } else if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, xnb32<D>(__builtin_bit_cast(int, v)));
+    } else {
+        typedef int i2v __attribute__((ext_vector_type(2)));
+        const i2v w = __builtin_bit_cast(i2v, v);
+        const i2v s = {xnb32<D>(w[0]), xnb32<D>(w[1])};
+        return __builtin_bit_cast(T, s);
+    }
+}
+
+// x1 bands in source form: t[R + d] of lane m is the coefficient with which node m feeds node
+// m - d, so sum_d b[n][d] v(n + d) = sum_d shift_d(t[d] v), and a line and an accumulated
+// derivative that share a shift share one DPP move.
+// o += sum_{d=-RA..RA} shift_d(ta[d] va + tb[d] vb), tb reaching RB <= RA.
+template <int RA, int RB, int D = -RA, typename T>
+__device__ __forceinline__ T xpair(const T* ta, const T* tb, T va, T vb, T o) {
+    if constexpr (D > RA) {
+        return o;
+    } else {
+        T t = ta[RA + D] * va;
+        if constexpr (D >= -RB && D <= RB) t = tb[RB + D] * vb + t;
+        o += xnb<D>(t);
+        if constexpr (sizeof(T) == 8) EQD_FENCE();  // one tap at a time: FP64 registers are scarce
+        return xpair<RA, RB, D + 1>(ta, tb, va, vb, o);
+    }
+}
+
+template <int RA, int RB, typename T>
+__device__ __forceinline__ void xpair4(const T* ta, const T* tb, const xv4<T>& va, const xv4<T>& vb, xv4<T>& o) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        o[e] = xpair<RA, RB>(ta, tb, va[e], vb[e], o[e]);
+        EQD_FENCE();
+    }
+}
+
+// Per-unit state shared by the plane steps of k_xi_pencil.
+template <typename T, int Z>
+struct XiUnit {
+    const PencilView<T>& op;
+    __amdgpu_buffer_rsrc_t rs;
+    int voff, NY, y, z0, nk;
+    const T* Tsy;
+    const T* Toy;
+    T cTs[4], cTo[4], cD1[4], cDs[4];  // x2 band taps of row y at dy = -2, -1, 1, 2
+    int rr[4];                         // rows y + dy, clamped
+    __device__ int line(int yy, int zz, int c) const { return ((yy + NY * zz) * 3 + c) * kXiBlock * (int)sizeof(T); }
+    __device__ xv4<T> ld(int yy, int zz, int c) const { return xload<T>(rs, voff, line(yy, zz, c)); }
+};
+
+// Terms of a plane that read row y (c0, c1, c2) and components 1 of rows y +- 1, 2 (v1, with
+// their D1y sum w1): phase A of xi_plane.
+template <typename T, int Z, int P>
+__device__ __forceinline__ void xi_terms_a(const XiUnit<T, Z>& U, const xv4<T>& c0, const xv4<T>& c1,
+                                           const xv4<T>& c2, const xv4<T> (&v1)[4], xv4<T> (&o)[Z][3],
+                                           xv4<T> (&a)[Z][3]) {
+    xv4<T> w1 = U.cDs[0] * v1[0];
+#pragma unroll
+    for (int j = 1; j < 4; ++j) w1 += U.cDs[j] * v1[j];
+#pragma unroll
+    for (int k = 0; k < Z; ++k) {
+        if (k >= U.nk) continue;
+        const int zk = U.z0 + k;
+        const int dz = P - kRyO - k;  // compile-time after unrolling
+        const T* Tsz = U.op.Ts + (size_t)zk * (2 * kRyS + 1) + kRyS;
+        const T* Toz = U.op.To + (size_t)zk * (2 * kRyO + 1) + kRyO;
+        const T* D1z = U.op.D1 + (size_t)zk * (2 * kRyO + 1) + kRyO;
+        if (dz == 0) {
+            // own lines: the centre taps of the x2 and x3 bands, and the x2 terms of component 1
+            const T cy0 = ucoef(U.Toy), cy1 = ucoef(U.Tsy), cz0 = ucoef(Toz), cz2 = ucoef(Tsz);
+            o[k][0] += (cy0 + cz0) * c0;
+            o[k][1] += (cy1 + cz0) * c1;
+            o[k][2] += (cy0 + cz2) * c2;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                o[k][1] += U.cTs[j] * v1[j];
+                a[k][0] += U.cD1[j] * v1[j];
+            }
+        } else if (dz >= -kRyO && dz <= kRyO) {
+            // x3 terms of row y and the mixed x2-x3 term of component 1 (the D1 rows of interior
+            // nodes have a zero centre tap, so dz = 0 adds nothing to the latter)
+            const T tS = ucoef(Tsz + dz), tO = ucoef(Toz + dz), tD = ucoef(D1z + dz);
+            o[k][2] += tS * c2;
+            a[k][0] += tD * c2;
+            o[k][0] += tO * c0;
+            a[k][2] += tD * c0;
+            o[k][1] += tO * c1;
+            o[k][2] += tD * w1;
+        } else if (dz >= -kRyS && dz <= kRyS) {
+            const T tS = ucoef(Tsz + dz);  // wide closure rows of the x3 band
+            if (tS != T(0)) o[k][2] += tS * c2;
+        }
+    }
+}
+
+// Terms of a plane that read components 2 (v2, D1y sum w2) and 0 (v0, on the pencils' own
+// planes) of rows y +- 1, 2: phase B of xi_plane.
+template <typename T, int Z, int P>
+__device__ __forceinline__ void xi_terms_b(const XiUnit<T, Z>& U, const xv4<T> (&v2)[4], const xv4<T> (&v0)[4],
+                                           xv4<T> (&o)[Z][3], xv4<T> (&a)[Z][3]) {
+    xv4<T> w2 = U.cDs[0] * v2[0];
+#pragma unroll
+    for (int j = 1; j < 4; ++j) w2 += U.cDs[j] * v2[j];
+#pragma unroll
+    for (int k = 0; k < Z; ++k) {
+        if (k >= U.nk) continue;
+        const int zk = U.z0 + k;
+        const int dz = P - kRyO - k;
+        if (dz == 0) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                o[k][0] += U.cTo[j] * v0[j];
+                a[k][1] += U.cD1[j] * v0[j];
+                o[k][2] += U.cTo[j] * v2[j];
+            }
+        } else if (dz >= -kRyO && dz <= kRyO) {
+            const T tD = ucoef(U.op.D1 + (size_t)zk * (2 * kRyO + 1) + kRyO + dz);
+            o[k][1] += tD * w2;
+        }
+    }
+}
+
+// One x3 plane zp = z0 - 2 + P of a unit: load its lines (row y: every component; rows y +- 1, 2:
+// components 1 and 2, and component 0 when a pencil of the unit lies on the plane) and apply
+// them to each pencil k of the unit, the x3 offset dz = P - 2 - k being a compile-time constant.
+// The (lam+mu) D1y sums of components 1 and 2 are formed once per plane and D1z applies them
+// to each pencil (mixed x2-x3 term). The x1 bands of the pencils' own lines act at the end of
+// the unit (k_xi_pencil), together with the mixed x1 terms. FP64 loads the plane in two phases
+// (row y and component 1, then components 2 and 0) to halve the registers held by lines.
+template <typename T, int Z, int P>
+__device__ __forceinline__ void xi_plane(const XiUnit<T, Z>& U, xv4<T> (&o)[Z][3], xv4<T> (&a)[Z][3]) {
+    EQD_FENCE();
+    const int zp = U.z0 - kRyO + P;
+    if (zp < 0 || zp >= U.NY) return;
+    constexpr int KP = P - kRyO;  // the unit's pencil on this plane, if 0 <= KP < Z
+    const bool own = KP >= 0 && KP < Z && KP < U.nk;
+    const xv4<T> c0 = U.ld(U.y, zp, 0), c1 = U.ld(U.y, zp, 1), c2 = U.ld(U.y, zp, 2);
+    xv4<T> v1[4], v2[4], v0[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v1[j] = U.ld(U.rr[j], zp, 1);
+    if constexpr (sizeof(T) == 8) {
+        EQD_FENCE();
+        xi_terms_a<T, Z, P>(U, c0, c1, c2, v1, o, a);
+        EQD_FENCE();
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v2[j] = U.ld(U.rr[j], zp, 2);
+    if (own) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v0[j] = U.ld(U.rr[j], zp, 0);
+    }
+    EQD_FENCE();
+    if constexpr (sizeof(T) == 4) xi_terms_a<T, Z, P>(U, c0, c1, c2, v1, o, a);
+    xi_terms_b<T, Z, P>(U, v2, v0, o, a);
+}
+
+template <typename T, int Z, int... P>
+__device__ __forceinline__ void xi_planes(const XiUnit<T, Z>& U, xv4<T> (&o)[Z][3], xv4<T> (&a)[Z][3],
+                                          std::integer_sequence<int, P...>) {
+    (xi_plane<T, Z, P>(U, o, a), ...);
+}
+
+template <typename T, int MODE, int Z>
+constexpr int xi_waves() {
+    return sizeof(T) == 8 ? 2 : EQD_XI_WAVES;
+}
+
+// MODE 0: q = A in and the weighted partial in.q per column (T = float).
+// MODE 1: reliable-update residual r = b - A x (T = double), stored as R for the flagged
+//         columns, with the weighted partial r.r (r.(dinv r) with a Jacobi dinv).
+// Grid: (G, tiles) with G a multiple of 64; block g takes units g, g + G, ...
+template <typename T, typename R, int MODE, int Z>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(xi_waves<T, MODE, Z>()))) void k_xi_pencil(
+    PencilView<T> op, const T* __restrict__ in, T* __restrict__ q, const double* __restrict__ b,
+    R* __restrict__ rout, const ColState* __restrict__ cs, const int* __restrict__ tile_on,
+    double* __restrict__ partial, const double* __restrict__ dinv) {
+    int tile, g;
+    tile_group(tile, g);
+    if (!tile_on[tile]) return;
+    constexpr int WS = 2 * kRxS + 1, WO = 2 * kRxO + 1;
+    __shared__ T xs[16 * WS], xo[16 * WO], xm[16 * WO];
+    __shared__ double sm[64];
+    // source form of the x1 bands: entry (m, d) = band row m - d, tap d (0 off the 16 rows)
+    for (int i = threadIdx.x; i < 16 * WS; i += kBlock) {
+        const int m = i / WS, d = i % WS - kRxS, r = m - d;
+        xs[i] = (r >= 0 && r < 16) ? op.AXs[r * WS + kRxS + d] : T(0);
+    }
+    for (int i = threadIdx.x; i < 16 * WO; i += kBlock) {
+        const int m = i / WO, d = i % WO - kRxO, r = m - d;
+        xo[i] = (r >= 0 && r < 16) ? op.AXo[r * WO + kRxO + d] : T(0);
+        xm[i] = (r >= 0 && r < 16) ? op.MX[r * WO + kRxO + d] : T(0);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int n = lane & 15, cq = lane >> 4;
+    const int NX = op.NX, NY = op.NY;
+    const bool live = n < NX;
+    const size_t tfl = xi_tile_elems(NY);
+    const size_t off = (size_t)tile * tfl;
+    const int col0 = 16 * wv + 4 * cq;  // the lane's columns col0 .. col0 + 3
+    const int lo = ((4 * wv + cq) * 16 + n) * 4;  // their element offset in a (line, component) block
+    bool mine[4] = {true, true, true, true};
+    if constexpr (MODE == 1) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) mine[i] = cs[(size_t)tile * 64 + col0 + i].rel_flag != 0;
+    }
+    const T mw = (live && n == op.mface) ? T(2) : T(1);
+    const T hx = live ? op.Hx[n] : T(0);
+    XiUnit<T, Z> U{op, eqd_rsrc(in + off, (uint32_t)(tfl * sizeof(T))), lo * (int)sizeof(T), NY, 0, 0, 0,
+                   nullptr, nullptr, {}, {}, {}, {}, {}};
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int nzg = (NY + Z - 1) / Z, nunits = NY * nzg;
+    for (int u0 = g; u0 < nunits; u0 += (int)gridDim.x) {
+        const int u = uni(u0);
+        const int y = u % NY, z0 = (u / NY) * Z;
+        U.y = y;
+        U.z0 = z0;
+        U.nk = min(Z, NY - z0);
+        U.Tsy = op.Ts + (size_t)y * (2 * kRyS + 1) + kRyS;
+        U.Toy = op.To + (size_t)y * (2 * kRyO + 1) + kRyO;
+        const T* D1y = op.D1 + (size_t)y * (2 * kRyO + 1) + kRyO;
+        const T* D1sy = op.D1s + (size_t)y * (2 * kRyO + 1) + kRyO;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int dy = j < 2 ? j - 2 : j - 1;  // -2, -1, 1, 2
+            U.cTs[j] = ucoef(U.Tsy + dy);
+            U.cTo[j] = ucoef(U.Toy + dy);
+            U.cD1[j] = ucoef(D1y + dy);
+            U.cDs[j] = ucoef(D1sy + dy);
+            U.rr[j] = rclamp(y + dy, NY);
+        }
+        xv4<T> o[Z][3], a[Z][3];
+#pragma unroll
+        for (int k = 0; k < Z; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                o[k][c] = xv4<T>{T(0), T(0), T(0), T(0)};
+                a[k][c] = xv4<T>{T(0), T(0), T(0), T(0)};
+            }
+        // planes z0 - 2 .. z0 + Z + 1
+        xi_planes<T, Z>(U, o, a, std::make_integer_sequence<int, Z + 2 * kRyO>{});
+        EQD_FENCE();
+        // closure reaches beyond the interior bands: x2 rows y +- 3, 4 (Ts of component 1) on the
+        // pencils' planes and x3 planes z0 - 4, z0 - 3, z0 + Z + 2, z0 + Z + 3 (Ts of component 2)
+        const int nk = U.nk;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int dy = j < 2 ? j - 4 : j + 1;  // -4, -3, 3, 4
+            const T c = ucoef(U.Tsy + dy);
+            if (c != T(0)) {
+                const int yy = rclamp(y + dy, NY);
+#pragma unroll
+                for (int k = 0; k < Z; ++k)
+                    if (k < nk) o[k][1] += c * U.ld(yy, z0 + k, 1);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int zp = j < 2 ? z0 - 4 + j : z0 + Z + j;  // z0 - 4, z0 - 3, z0 + Z + 2, z0 + Z + 3
+            if (zp < 0 || zp >= NY) continue;
+            bool any = false;
+            T ck[Z];
+#pragma unroll
+            for (int k = 0; k < Z; ++k) {
+                const int dz = zp - (z0 + k);
+                ck[k] = (k < nk && dz >= -kRyS && dz <= kRyS)
+                            ? ucoef(op.Ts + (size_t)(z0 + k) * (2 * kRyS + 1) + kRyS + dz)
+                            : T(0);
+                any = any || ck[k] != T(0);
+            }
+            if (any) {
+                const xv4<T> v = U.ld(y, zp, 2);
+#pragma unroll
+                for (int k = 0; k < Z; ++k) o[k][2] += ck[k] * v;
+            }
+        }
+        // the own lines again (cache hits): their x1 bands together with (lam+mu) D1x of the
+        // accumulated x2 / x3 derivatives; weights, stores and the partial dot
+        const T hy = op.Hy[y];
+#pragma unroll
+        for (int k = 0; k < Z; ++k) {
+            if (k >= nk) continue;
+            const int zk = z0 + k;
+            const xv4<T> p0 = U.ld(y, zk, 0), p1 = U.ld(y, zk, 1), p2 = U.ld(y, zk, 2);
+            // band offsets made opaque here, so the coefficients are read from LDS where they are
+            // used instead of being held in registers across the unit loop
+            int os = n * WS, oo = n * WO;
+            asm volatile("" : "+v"(os), "+v"(oo));
+            xpair4<kRxS, kRxO>(xs + os, xm + oo, p0, a[k][0], o[k][0]);
+            xpair4<kRxO, kRxO>(xo + oo, xm + oo, p1, a[k][1], o[k][1]);
+            xpair4<kRxO, kRxO>(xo + oo, xm + oo, p2, a[k][2], o[k][2]);
+            const T hw = -(hx * (hy * op.Hy[zk]));
+            const size_t e = off + xi_elem(y + NY * zk, 0, 0, 0) + lo;
+            double d[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const xv4<T> v = hw * o[k][c];
+                if constexpr (MODE == 0) {
+                    const xv4<T> pv = c == 0 ? p0 : (c == 1 ? p1 : p2);
+                    // padding lanes store zeros: every 256-byte segment of q is written whole
+                    __builtin_nontemporal_store(live ? v : xv4<T>{T(0), T(0), T(0), T(0)},
+                                                (xv4<T>*)(q + e + c * kXiBlock));
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) d[i] += (double)pv[i] * (double)v[i];
+                } else {
+                    const double dz = dinv ? dinv[(size_t)(n + NX * (y + NY * zk)) * 3 + c] : 1.0;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const double rv = b[e + c * kXiBlock + i] - (double)v[i];
+                        if (live && mine[i]) rout[e + c * kXiBlock + i] = (R)rv;
+                        d[i] += rv * (dz * rv);
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i] += (double)mw * d[i];
+        }
+    }
+    // per column: sum over the 16 node lanes of a row (fixed butterfly), one slot per column
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double s = live ? acc[i] : 0.0;
+        s += __shfl_xor(s, 8, 16);
+        s += __shfl_xor(s, 4, 16);
+        s += __shfl_xor(s, 2, 16);
+        s += __shfl_xor(s, 1, 16);
+        if (n == 0) sm[col0 + i] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) partial[((size_t)tile * gridDim.x + g) * 64 + threadIdx.x] = sm[threadIdx.x];
+}
+
+// Slot geometry of the vector kernels: thread t visits 16-byte slots v = g 256 + t + k G 256 of
+// its tile, all with node m = t % 16 and column quad t / 16 (in FP32 slots).
+struct XiSlot {
+    int m, cq;
+};
+__device__ __forceinline__ XiSlot xi_slot() { return XiSlot{(int)(threadIdx.x & 15), (int)(threadIdx.x >> 4)}; }
+
+// Per-column partials of a vector kernel: the 16 node threads of a column quad (one 16-lane
+// row of a wave) in a fixed butterfly, then partial[tile][g][64].
+__device__ __forceinline__ void xi_partial(double (&acc)[4], double* __restrict__ partial, int tile, int g) {
+    const int cq = threadIdx.x >> 4, m = threadIdx.x & 15;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double s = acc[i];
+        s += __shfl_xor(s, 8, 16);
+        s += __shfl_xor(s, 4, 16);
+        s += __shfl_xor(s, 2, 16);
+        s += __shfl_xor(s, 1, 16);
+        if (m == 0) partial[((size_t)tile * gridDim.x + g) * 64 + 4 * cq + i] = s;
+    }
+}
+
+// r -= alpha q and the weighted partial r.r (r.(dinv r) with a Jacobi dinv), FP32 vectors.
+template <bool PRE>
+__global__ __launch_bounds__(kBlock) void k_xi_update(int NX, int NY, int mface, float* __restrict__ r,
+                                                      const float* __restrict__ q, const ColState* __restrict__ cs,
+                                                      const int* __restrict__ tile_on, double* __restrict__ partial,
+                                                      const float* __restrict__ dinv) {
+    using V = xv4<float>;
+    int tile, g;
+    tile_group(tile, g);
+    const XiSlot sl = xi_slot();
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (tile_on[tile]) {
+        // padding nodes (m >= NX) take part with zeros, so every 256-byte segment is written whole
+        const bool pad = sl.m >= NX;
+        float a[4];
+        bool act[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const ColState& st = cs[(size_t)tile * 64 + 4 * sl.cq + j];
+            act[j] = st.status == kActive;
+            a[j] = pad ? 0.0f : (float)st.alpha;
+        }
+        const double w = sl.m == mface ? 2.0 : 1.0;
+        const size_t off = xi_tile_elems(NY) * tile;
+        V* rv = reinterpret_cast<V*>(r + off);
+        const V* qv = reinterpret_cast<const V*>(q + off);
+        const size_t nv = xi_tile_elems(NY) / 4;
+        for (size_t v = (size_t)g * kBlock + threadIdx.x; v < nv; v += (size_t)gridDim.x * kBlock) {
+            V rr = __builtin_nontemporal_load(rv + v);
+            const V qq = __builtin_nontemporal_load(qv + v);
+            float dz = 1.0f;
+            if constexpr (PRE) {
+                const int bl = (int)(v >> 8), c = bl % 3, yz = bl / 3;
+                if (!pad) dz = dinv[(size_t)(sl.m + NX * yz) * 3 + c];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (act[j]) {
+                    const float rn = pad ? 0.0f : rr[j] - a[j] * qq[j];
+                    rr[j] = rn;
+                    acc[j] += w * ((double)rn * (double)(dz * rn));
+                }
+            }
+            __builtin_nontemporal_store(rr, rv + v);
+        }
+    }
+    xi_partial(acc, partial, tile, g);
+}
+
+// x += alpha p (columns updated this iteration) and p = r + beta p (dinv r + beta p with a
+// Jacobi dinv) for the columns still active, FP32 vectors.
+template <bool PRE>
+__global__ __launch_bounds__(kBlock) void k_xi_pupdate(int NX, int NY, float* __restrict__ x, float* __restrict__ p,
+                                                       const float* __restrict__ r, const ColState* __restrict__ cs,
+                                                       const float* __restrict__ dinv) {
+    using V = xv4<float>;
+    int tile, g;
+    tile_group(tile, g);
+    const XiSlot sl = xi_slot();
+    const bool pad = sl.m >= NX;  // padding nodes write zeros: whole 256-byte segments
+    float a[4], be[4];
+    bool xu[4], pu[4];
+    bool anyx = false, anyp = false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const ColState& st = cs[(size_t)tile * 64 + 4 * sl.cq + j];
+        xu[j] = st.xpend != 0;
+        pu[j] = st.status == kActive;
+        a[j] = (float)st.alpha;
+        be[j] = (float)st.beta;
+        anyx |= xu[j];
+        anyp |= pu[j];
+    }
+    if (!anyx && !anyp) return;
+    const size_t off = xi_tile_elems(NY) * tile;
+    V* xv = reinterpret_cast<V*>(x + off);
+    V* pv = reinterpret_cast<V*>(p + off);
+    const V* rv = reinterpret_cast<const V*>(r + off);
+    const size_t nv = xi_tile_elems(NY) / 4;
+    for (size_t v = (size_t)g * kBlock + threadIdx.x; v < nv; v += (size_t)gridDim.x * kBlock) {
+        V pp = __builtin_nontemporal_load(pv + v);
+        if (anyx) {
+            V xx = __builtin_nontemporal_load(xv + v);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (xu[j]) xx[j] = pad ? 0.0f : xx[j] + a[j] * pp[j];
+            __builtin_nontemporal_store(xx, xv + v);
+        }
+        if (anyp) {
+            const V rr = __builtin_nontemporal_load(rv + v);
+            float dz = 1.0f;
+            if constexpr (PRE) {
+                const int bl = (int)(v >> 8), c = bl % 3, yz = bl / 3;
+                if (!pad) dz = dinv[(size_t)(sl.m + NX * yz) * 3 + c];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (pu[j]) pp[j] = pad ? 0.0f : (PRE ? dz * rr[j] : rr[j]) + be[j] * pp[j];
+            __builtin_nontemporal_store(pp, pv + v);
+        }
+    }
+}
+
+// Reliable update, part 1: x64 += xc, xc = 0 for the flagged columns (every column when all !=
+// 0; exactly the masked columns with a mask).
+__global__ __launch_bounds__(kBlock) void k_xi_xaccum(int NX, int NY, double* __restrict__ x, float* __restrict__ xc,
+                                                      const ColState* __restrict__ cs,
+                                                      const int* __restrict__ tile_on, int all,
+                                                      const int* __restrict__ mask) {
+    int tile, g;
+    tile_group(tile, g);
+    const XiSlot sl = xi_slot();  // padding nodes hold zeros in x and xc: they take part as is
+    bool f[4];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const size_t col = (size_t)tile * 64 + 4 * sl.cq + j;
+        f[j] = mask ? mask[col] != 0 : (all || (tile_on[tile] && cs[col].rel_flag));
+        any |= f[j];
+    }
+    if (!any) return;
+    const size_t off = xi_tile_elems(NY) * tile;
+    const size_t nv = xi_tile_elems(NY) / 4;
+    for (size_t v = (size_t)g * kBlock + threadIdx.x; v < nv; v += (size_t)gridDim.x * kBlock) {
+        const size_t e = off + 4 * v;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (f[j]) {
+                x[e + j] += (double)xc[e + j];
+                xc[e + j] = 0.0f;
+            }
+        }
+    }
+}
+
+// Weighted partial dot of b with itself per column (b.(dinv b) with a Jacobi dinv), FP64.
+__global__ __launch_bounds__(kBlock) void k_xi_selfdot(int NX, int NY, int mface, const double* __restrict__ v,
+                                                       double* __restrict__ partial,
+                                                       const double* __restrict__ dinv) {
+    int tile, g;
+    tile_group(tile, g);
+    const XiSlot sl = xi_slot();
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (sl.m < NX) {
+        const double w = sl.m == mface ? 2.0 : 1.0;
+        const size_t off = xi_tile_elems(NY) * tile;
+        const size_t nv = xi_tile_elems(NY) / 4;
+        for (size_t s = (size_t)g * kBlock + threadIdx.x; s < nv; s += (size_t)gridDim.x * kBlock) {
+            const int bl = (int)(s >> 8), c = bl % 3, yz = bl / 3;
+            const double dz = dinv ? dinv[(size_t)(sl.m + NX * yz) * 3 + c] : 1.0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double vv = v[off + 4 * s + j];
+                acc[j] += w * (vv * (dz * vv));
+            }
+        }
+    }
+    xi_partial(acc, partial, tile, g);
+}
+
+// b = H P (D+SAT) chi(s) and the Krylov start (x = 0, xc = 0, r = b, p = b or dinv b) for the
+// masked (or all) columns, as k_rhs, in the x1-inner layout: thread slots of four columns.
+__global__ __launch_bounds__(kBlock) void k_xi_rhs(RhsView rv, const double* __restrict__ fields, int nslots,
+                                                   double* __restrict__ b, double* __restrict__ x,
+                                                   float* __restrict__ r, float* __restrict__ p,
+                                                   float* __restrict__ xc, const int* __restrict__ mask,
+                                                   const double* __restrict__ dinv) {
+    int tile, g;
+    tile_group(tile, g);
+    const XiSlot sl = xi_slot();
+    const int NX = rv.NX, NY = rv.NY, n23 = rv.n23;
+    bool on[4];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = tile * 64 + 4 * sl.cq + j;
+        on[j] = !mask || mask[col];
+        any |= on[j];
+    }
+    if (!any) return;
+    const size_t off = xi_tile_elems(NY) * tile;
+    const size_t nv = xi_tile_elems(NY) / 4;
+    const int m = sl.m;
+    for (size_t s = (size_t)g * kBlock + threadIdx.x; s < nv; s += (size_t)gridDim.x * kBlock) {
+        const int bl = (int)(s >> 8), c = bl % 3, yz = bl / 3;
+        const int y = yz % NY, z = yz / NY, iy = y + 1, iz = z + 1;
+        const size_t e = off + 4 * s;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (!on[j]) continue;
+            const int col = tile * 64 + 4 * sl.cq + j;
+            double val = 0.0;
+            if (m < NX && col < nslots) {
+                const double* s2 = fields + (size_t)col * 2 * n23 * n23;
+                const double* s3 = s2 + (size_t)n23 * n23;
+                const double H = rv.Hx[m] * rv.Hy[y] * rv.Hy[z];
+                if (c > 0) {
+                    const double jt = rv.Jt[m];
+                    if (jt != 0.0) val = H * jt * (c == 1 ? s2 : s3)[iy + (size_t)n23 * iz];
+                } else {
+                    const double jn = rv.Jn[m];
+                    if (jn != 0.0) {
+                        double d = 0.0;
+                        const int sy = rv.D1full.start[y], wy = rv.D1full.width[y];
+                        for (int t = 0; t < wy; ++t)
+                            d += rv.D1full.c[(size_t)y * kTabW + t] * s2[(sy + t) + (size_t)n23 * iz];
+                        const int sz = rv.D1full.start[z], wz = rv.D1full.width[z];
+                        for (int t = 0; t < wz; ++t)
+                            d += rv.D1full.c[(size_t)z * kTabW + t] * s3[iy + (size_t)n23 * (sz + t)];
+                        val = H * jn * d;
+                    }
+                }
+            }
+            b[e + j] = val;
+            x[e + j] = 0.0;
+            r[e + j] = (float)val;
+            p[e + j] = (m < NX && dinv) ? (float)(dinv[(size_t)(m + NX * yz) * 3 + c] * val) : (float)val;
+            xc[e + j] = 0.0f;
+        }
+    }
+}
+
+}  // namespace eqd
