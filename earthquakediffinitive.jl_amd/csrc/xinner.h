@@ -89,7 +89,9 @@ __device__ __forceinline__ T xpair(const T* ta, const T* tb, T va, T vb, T o) {
         T t = ta[RA + D] * va;
         if constexpr (D >= -RB && D <= RB) t = tb[RB + D] * vb + t;
         o += xnb<D>(t);
-        if constexpr (sizeof(T) == 8) EQD_FENCE();  // one tap at a time: FP64 registers are scarce
+        // FP64: the running sum is materialised after every tap, so each shifted product is
+        // consumed at once instead of all of them being held for a deferred sum (register spills)
+        if constexpr (sizeof(T) == 8) asm volatile("" : "+v"(o));
         return xpair<RA, RB, D + 1>(ta, tb, va, vb, o);
     }
 }
@@ -231,7 +233,7 @@ __device__ __forceinline__ void xi_planes(const XiUnit<T, Z>& U, xv4<T> (&o)[Z][
 
 template <typename T, int MODE, int Z>
 constexpr int xi_waves() {
-    return sizeof(T) == 8 ? 2 : EQD_XI_WAVES;
+    return sizeof(T) == 8 ? 3 : EQD_XI_WAVES;
 }
 
 // MODE 0: q = A in and the weighted partial in.q per column (T = float).
