@@ -105,6 +105,34 @@ __device__ __forceinline__ void xpair4(const T* ta, const T* tb, const xv4<T>& v
     }
 }
 
+// FP32 x1 bands on the matrix cores. For the 16 columns of a wave, out^T (16 columns x 16 nodes)
+// += V^T (16 x 16) Band^T is four v_mfma_f32_16x16x4f32 steps per band: the accumulator lane
+// layout (lane = node + 16 (column / 4), register = column % 4) is the x1-inner one, so the
+// accumulated x2 / x3 terms enter as C directly. The A operand (lane = column + 16 k, the value
+// of node 4 s + k at step s) comes from a transpose through a per-wave LDS tile, in which node
+// n sits at position 4 (n % 4) + n / 4 of its column's row, so each lane reads its four steps
+// with one 16-byte load. The B operand (lane = node j + 16 k: Band[j][4 s + k]) is constant.
+constexpr int kXtRow = 20;  // floats per column row of the transpose tile (16 + 4: bank spread)
+
+__device__ __forceinline__ float xband_at(const float* band, int R, int j, int m) {
+    const int d = m - j;
+    return (d >= -R && d <= R) ? band[j * (2 * R + 1) + R + d] : 0.0f;
+}
+
+// A operands of one input (lane layout: node n, column quad cq, four columns in registers).
+__device__ __forceinline__ xv4<float> xt_transpose(float* tile, const xv4<float>& v, int n, int cq, int lane) {
+    const int pos = 4 * (n & 3) + (n >> 2);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tile[(4 * cq + r) * kXtRow + pos] = v[r];
+    return *reinterpret_cast<const xv4<float>*>(tile + (lane & 15) * kXtRow + 4 * (lane >> 4));
+}
+
+__device__ __forceinline__ xv4<float> xt_mfma(const xv4<float>& at, const float (&bt)[4], xv4<float> c) {
+#pragma unroll
+    for (int st = 0; st < 4; ++st) c = __builtin_amdgcn_mfma_f32_16x16x4f32(at[st], bt[st], c, 0, 0, 0);
+    return c;
+}
+
 // Per-unit state shared by the plane steps of k_xi_pencil.
 template <typename T, int Z>
 struct XiUnit {
@@ -250,6 +278,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(xi_waves
     if (!tile_on[tile]) return;
     constexpr int WS = 2 * kRxS + 1, WO = 2 * kRxO + 1;
     __shared__ T xs[16 * WS], xo[16 * WO], xm[16 * WO];
+    // per-wave transpose tiles of the matrix-core x1 bands (FP32): six inputs per pencil
+    __shared__ __attribute__((aligned(16))) float xtile[sizeof(T) == 4 ? 4 * 6 * 16 * kXtRow : 1];
     __shared__ double sm[64];
     // source form of the x1 bands: entry (m, d) = band row m - d, tap d (0 off the 16 rows)
     for (int i = threadIdx.x; i < 16 * WS; i += kBlock) {
@@ -266,6 +296,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(xi_waves
     const int n = lane & 15, cq = lane >> 4;
     const int NX = op.NX, NY = op.NY;
     const bool live = n < NX;
+    float bAs[4] = {}, bAo[4] = {}, bMx[4] = {};  // B operands: Band[lane % 16][4 s + lane / 16]
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            const int m = 4 * st + (lane >> 4);
+            bAs[st] = xband_at(op.AXs, kRxS, n, m);
+            bAo[st] = xband_at(op.AXo, kRxO, n, m);
+            bMx[st] = xband_at(op.MX, kRxO, n, m);
+        }
+    }
     const size_t tfl = xi_tile_elems(NY);
     const size_t off = (size_t)tile * tfl;
     const int col0 = 16 * wv + 4 * cq;  // the lane's columns col0 .. col0 + 3
@@ -353,13 +393,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(xi_waves
             if (k >= nk) continue;
             const int zk = z0 + k;
             const xv4<T> p0 = U.ld(y, zk, 0), p1 = U.ld(y, zk, 1), p2 = U.ld(y, zk, 2);
-            // band offsets made opaque here, so the coefficients are read from LDS where they are
-            // used instead of being held in registers across the unit loop
-            int os = n * WS, oo = n * WO;
-            asm volatile("" : "+v"(os), "+v"(oo));
-            xpair4<kRxS, kRxO>(xs + os, xm + oo, p0, a[k][0], o[k][0]);
-            xpair4<kRxO, kRxO>(xo + oo, xm + oo, p1, a[k][1], o[k][1]);
-            xpair4<kRxO, kRxO>(xo + oo, xm + oo, p2, a[k][2], o[k][2]);
+            if constexpr (sizeof(T) == 4) {
+                float* xt = xtile + wv * 6 * 16 * kXtRow;
+                const xv4<float> t0 = xt_transpose(xt, p0, n, cq, lane);
+                const xv4<float> t1 = xt_transpose(xt + 16 * kXtRow, p1, n, cq, lane);
+                const xv4<float> t2 = xt_transpose(xt + 2 * 16 * kXtRow, p2, n, cq, lane);
+                const xv4<float> u0 = xt_transpose(xt + 3 * 16 * kXtRow, a[k][0], n, cq, lane);
+                const xv4<float> u1 = xt_transpose(xt + 4 * 16 * kXtRow, a[k][1], n, cq, lane);
+                const xv4<float> u2 = xt_transpose(xt + 5 * 16 * kXtRow, a[k][2], n, cq, lane);
+                o[k][0] = xt_mfma(u0, bMx, xt_mfma(t0, bAs, o[k][0]));
+                o[k][1] = xt_mfma(u1, bMx, xt_mfma(t1, bAo, o[k][1]));
+                o[k][2] = xt_mfma(u2, bMx, xt_mfma(t2, bAo, o[k][2]));
+            } else {
+                // band offsets made opaque here, so the coefficients are read from LDS where they are
+                // used instead of being held in registers across the unit loop
+                int os = n * WS, oo = n * WO;
+                asm volatile("" : "+v"(os), "+v"(oo));
+                xpair4<kRxS, kRxO>(xs + os, xm + oo, p0, a[k][0], o[k][0]);
+                xpair4<kRxO, kRxO>(xo + oo, xm + oo, p1, a[k][1], o[k][1]);
+                xpair4<kRxO, kRxO>(xo + oo, xm + oo, p2, a[k][2], o[k][2]);
+            }
             const T hw = -(hx * (hy * op.Hy[zk]));
             const size_t e = off + xi_elem(y + NY * zk, 0, 0, 0) + lo;
             double d[4] = {0.0, 0.0, 0.0, 0.0};
