@@ -668,17 +668,19 @@ def main():
         # whole-iteration metric of the survey: 10 vector passes x DOF x 4 B per column-iteration
         col_iters = rep["iterations"] / max(1, rep["solves"]) * ncols
         survey_gbs = 10 * dof * 4 * col_iters / (ms_per_step / 1e3)
-        # x1 lines of 9 to 16 nodes: the x1-inner layout and its x3-marching stencil; longer lines:
-        # full-line (one 16-node segment) or segmented pencils
+        # x1 lines of 9 to 16 nodes: the x1-inner layout and its x3-marching stencil; 17 to 64
+        # nodes: the x1-inner layout with one wave per 16-node segment; longer lines: segmented
+        # node-major pencils
         nx = 2 * g.n1 - 3
-        full = "true" if nx <= 16 else "false"
         tk = eqd.solver_report(fet.rs)["tile_k"]
         if tk != 64:
             stencil_name = f"k_apply_dot<float,{tk}>"
         elif 9 <= nx <= 16:
             stencil_name = "k_xi_pencil<float,float,0,3>"
+        elif nx <= 64:
+            stencil_name = "k_xl_pencil<float,float,0,3>"
         else:
-            stencil_name = f"k_pencil<float,float,0,{full}>"
+            stencil_name = "k_pencil<float,float,0,false>"
         out["roofline"] = {"bound": "hbm", "kernel": stencil_name,
                            "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK, 4), "traffic": measured_traffic(stencil_name),

@@ -40,9 +40,9 @@ void validate_grid(const eqd_grid_desc& g) {
     // byte offsets: the largest one (FP64, 3 components per stored node) must stay below 2^31 B.
     const int64_t NN = (int64_t)(2 * g.n1 - 3) * (g.n23 - 2) * (g.n23 - 2);
     const int64_t tile_bytes = NN * 3 * 64 * (int64_t)sizeof(double);
-    // x1-inner tiles (x1 lines of 9 to 16 nodes) pad the lines to 16 nodes
-    const int64_t NXm = 2 * g.n1 - 3;
-    const int64_t xi_bytes = (NXm >= 9 && NXm <= 16) ? (int64_t)(g.n23 - 2) * (g.n23 - 2) * 3 * 1024 * 8 : 0;
+    // x1-inner tiles (x1 lines of 9 to 64 nodes) pad the lines (to 16 nodes, or a multiple of 8)
+    const int NXm = 2 * g.n1 - 3;
+    const int64_t xi_bytes = (NXm >= 9 && NXm <= 64) ? (int64_t)xi_tile_elems(NXm, g.n23 - 2) * 8 : 0;
     if (tile_bytes > (int64_t)INT32_MAX || xi_bytes > (int64_t)INT32_MAX)
         throw Error(EQD_ERR_GRID, "grid too large: " + std::to_string(NN) + " stored nodes make a 64-column FP64 tile of " +
                                       std::to_string(tile_bytes) + " B, above the 2^31 B a tile vector may span (at most " +
@@ -124,7 +124,7 @@ struct eqd_op {
     // segments with x1 halos (p64's bands are padded to 16 rows, the pencil reads 15)
     PencilView<double> p64w{};
     bool p64w_ok = false;
-    // x1 lines of at most 16 nodes: the 64-column tiles of the mixed-precision engine use the
+    // x1 lines of 9 to 64 nodes: the 64-column tiles of the mixed-precision engine use the
     // x1-inner layout and its kernels (xinner.h)
     bool xi = false;
     int G = kGroups;  // workgroups per tile of the node-parallel kernels (set at creation)
@@ -316,7 +316,7 @@ void eqd::host::solver_fault_dims(const eqd_solver* s, int& nf, int& n2f, int& n
 namespace {
 
 size_t bytes_per_column(const eqd_op& op, int prec) {
-    const size_t n = (prec == EQD_PREC_MIXED && op.xi) ? xi_tile_elems(op.tb.NY) / 64 : (size_t)op.NN * 3;
+    const size_t n = (prec == EQD_PREC_MIXED && op.xi) ? xi_tile_elems(op.tb.NX, op.tb.NY) / 64 : (size_t)op.NN * 3;
     const size_t t = prec == EQD_PREC_MIXED ? 4 : 8;
     return n * (8 + 8 + 3 * t + (prec == EQD_PREC_MIXED ? 4 : 0)) + 2 * (size_t)op.g.n23 * op.g.n23 * 8;
 }
@@ -331,7 +331,7 @@ void ensure_ws(eqd_solver* s, int K, int ntiles, int prec) {
     w->K = K;
     w->ntiles = ntiles;
     w->prec = prec;
-    w->nvec = (K == 64 && prec == EQD_PREC_MIXED && op.xi) ? (size_t)ntiles * xi_tile_elems(op.tb.NY)
+    w->nvec = (K == 64 && prec == EQD_PREC_MIXED && op.xi) ? (size_t)ntiles * xi_tile_elems(op.tb.NX, op.tb.NY)
                                                            : (size_t)ntiles * op.NN * 3 * K;
     const size_t slots = (size_t)ntiles * K;
     const size_t t = prec == EQD_PREC_MIXED ? sizeof(float) : sizeof(double);
@@ -417,9 +417,14 @@ struct Engine {
                 // that no neighbour-line load falls past the line's end (stencil 559 -> 535 us)
                 if constexpr (std::is_same<T, float>::value) {
                     if (xi(s)) {
-                        hipLaunchKernelGGL((k_xi_pencil<float, float, 0, EQD_XI_Z>), grid, dim3(kBlock), 0, st, op.p32,
-                                           p, q, (const double*)nullptr, (float*)nullptr, cs, on, part,
-                                           (const double*)nullptr);
+                        if (op.tb.NX <= 16)
+                            hipLaunchKernelGGL((k_xi_pencil<float, float, 0, EQD_XI_Z>), grid, dim3(kBlock), 0, st,
+                                               op.p32, p, q, (const double*)nullptr, (float*)nullptr, cs, on, part,
+                                               (const double*)nullptr);
+                        else
+                            hipLaunchKernelGGL((k_xl_pencil<float, float, 0, EQD_XI_Z>), grid, dim3(kBlock), 0, st,
+                                               op.p32, p, q, (const double*)nullptr, (float*)nullptr, cs, on, part,
+                                               (const double*)nullptr);
                         return;
                     }
                 }
@@ -518,8 +523,11 @@ struct Engine {
                 {
                     const double* x64 = w.x.as<double>();
                     const double* b64 = w.b.as<double>();
-                    if (xi(s))
+                    if (xi(s) && op.tb.NX <= 16)
                         hipLaunchKernelGGL((k_xi_pencil<double, T, 1, 1>), grid, dim3(kBlock), 0, st, op.p64, x64,
+                                           (double*)nullptr, b64, r, cs, rel, part, dinv64);
+                    else if (xi(s))
+                        hipLaunchKernelGGL((k_xl_pencil<double, T, 1, 1>), grid, dim3(kBlock), 0, st, op.p64, x64,
                                            (double*)nullptr, b64, r, cs, rel, part, dinv64);
                     else if (op.p64.nseg == 1)
                         hipLaunchKernelGGL((k_pencil<double, T, 1, true>), grid, dim3(kBlock),
@@ -1341,8 +1349,9 @@ eqd_status eqd_operator_create(const eqd_grid_desc* g, eqd_op** out) {
         op->p32 = op->pview<float>();
         {
             const char* sv = std::getenv("EQD_STENCIL");
-            // 9 <= NX keeps the FP64 bands at 16 rows (two 8-node segments), as the kernels read them
-            op->xi = op->tb.NX >= 9 && op->tb.NX <= 16 && !(sv && std::strcmp(sv, "pencil") == 0);
+            // 9 <= NX keeps the FP64 bands of short lines at 16 rows (two 8-node segments), as the
+            // kernels read them; lines of more than 64 nodes keep the node-major pencils
+            op->xi = op->tb.NX >= 9 && op->tb.NX <= 64 && !(sv && std::strcmp(sv, "pencil") == 0);
         }
         if (op->tb.NX > pencil_s<double>() && op->tb.NX <= 15) {
             op->p64w = op->p64;

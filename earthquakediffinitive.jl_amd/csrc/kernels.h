@@ -827,6 +827,14 @@ __global__ __launch_bounds__(kBlock) void k_selfdot(int NN, int NX, int mface, c
     block_partial<K>(acc, partial, tile, g);
 }
 
+// x1-inner tiles (xinner.h): nodes per column quad of a (line, component) block. Lines of up to
+// 16 nodes pad to 16, longer ones to a multiple of 8, so every column quad spans whole 128-byte
+// lines of FP32 (and 256-byte lines of FP64).
+__host__ __device__ inline int xi_nxb(int NX) { return NX <= 16 ? 16 : (NX + 7) & ~7; }
+
+// Elements per tile vector of an x1-inner grid.
+__host__ __device__ inline size_t xi_tile_elems(int NX, int NY) { return (size_t)NY * NY * 3 * 64 * xi_nxb(NX); }
+
 // Traction change on Omega_f from the merged solution u and the slip fields:
 // dtau_c = 1/2 (T_c U- + T_c U+), U = P u + chi     (src/FaultResponse.jl:201-209)
 // Writes K[row, col] with row = (c-1)*nf + a + n2f*b (column-major, leading dim ldK).
@@ -846,10 +854,11 @@ __global__ void k_extract(ExtractView ev, int NN, const double* __restrict__ u, 
     const int iy = ev.i2lo + k % ev.n2f, iz = ev.i2lo + k / ev.n2f;
     const bool yin = iy >= 1 && iy <= n23 - 2, zin = iz >= 1 && iz <= n23 - 2;
     const double* f = fields + ((size_t)tile * K + lane) * 2 * n23 * n23;
-    const double* ut = u + (XI ? (size_t)NY * NY * 3 * 1024 : (size_t)NN * 3 * K) * tile;
+    const int B = xi_nxb(NX);  // x1 nodes per column quad of an x1-inner block
+    const double* ut = u + (XI ? xi_tile_elems(NX, NY) : (size_t)NN * 3 * K) * tile;
     auto ld = [&](int m, int y, int z, int c) {
         if constexpr (XI)
-            return ut[((size_t)((y + NY * z) * 3 + c) << 10) + ((lane >> 2) << 6) + (m << 2) + (lane & 3)];
+            return ut[(size_t)((y + NY * z) * 3 + c) * 64 * B + (lane >> 2) * 4 * B + 4 * m + (lane & 3)];
         else
             return ut[((size_t)(m + NX * (y + NY * z)) * 3 + c) * K + lane];
     };

@@ -1,13 +1,13 @@
 // x1-inner tile layout and its kernels (64-column tiles of the mixed-precision engine on grids
-// whose merged x1 lines have at most 16 nodes: n1 <= 9, the dz = 50 m and the dz = 100 m,
-// L_normal = 800 m grids). Included by kernels.h after pencil.h.
+// whose merged x1 lines have 9 to 64 nodes). Included by kernels.h after pencil.h.
 //
 // Layout. A tile's vector holds, for every x2-x3 line (y, z) and component c, one block of
-// 16 column quads x 16 x1 nodes x 4 columns:
-//     element (m, y, z, c, column j) = ((y + NY z) 3 + c) 1024 + (j / 4) 64 + 4 m + j % 4,
-// x1 nodes m >= NX being zero padding. A 16-byte access holds four columns of one node, so
-// 16 lanes with consecutive nodes read 256 contiguous bytes, and a wave's load fetches a whole
-// (line, component) for 16 columns. The FP64 vectors b and x use the same element order.
+// 16 column quads x B x1 nodes x 4 columns, B = 16 for lines of at most 16 nodes and B = NX
+// rounded up to a multiple of 8 for longer ones (nodes m >= NX zero padding), so that every
+// column quad's run of nodes is whole 128-byte lines:
+//     element (m, y, z, c, column j) = ((y + NY z) 3 + c) 64 B + (j / 4) 4 B + 4 m + j % 4.
+// A 16-byte access holds four columns of one node, so 16 lanes with consecutive nodes read
+// 256 contiguous bytes. The FP64 vectors b and x use the same element order.
 //
 // Stencil. Lanes are (x1 node n = lane % 16, column quad lane / 16); wave w of a block takes the
 // columns 16 w .. 16 w + 15, so one 16-byte load per lane (32 bytes in FP64) fetches the line
@@ -16,9 +16,13 @@
 // unit of Z pencils (x2 row y, planes z0 .. z0 + Z - 1): a line loaded once serves every pencil
 // of the unit that needs it, and the mixed x2-x3 term is sum-factorised per plane.
 //
-// Vector kernels. A thread owns one 16-byte slot per visited block: node m = t % 16 and column
-// quad t / 16 are the same for every slot it visits (the grid strides are multiples of 256
-// slots), so the face weight, the padding test and the per-column sums are per thread.
+// Lines of more than 16 nodes (k_xl_pencil): the four waves of a block are the x1 segments of
+// 16 nodes of one 16-column slab, and the x1 bands read their neighbours across segments from
+// an LDS copy of the unit's lines.
+//
+// Vector kernels. A workgroup walks (line, component) blocks g, g + G, ...; thread t owns the
+// slots of column quad t / 16 and nodes t % 16 + 16 s, so its columns (hence its per-column
+// sums) are fixed.
 #pragma once
 #include <utility>
 
@@ -34,17 +38,22 @@
 #ifndef EQD_XI_WAVES
 #define EQD_XI_WAVES 3
 #endif
+// EQD_XI_SLAB: the four waves of a block take four units of one 16-column slab (slab-major
+// order over a tile, so the slab's x2-x3 planes stay in L2) instead of the four slabs of one
+// unit. EQD_XI_SPREAD: those four units G/4 apart instead of consecutive.
+#ifndef EQD_XI_SLAB
+#define EQD_XI_SLAB 1
+#endif
+#ifndef EQD_XI_SPREAD
+#define EQD_XI_SPREAD 0
+#endif
+// Units (x2 row y, x3 group of Z pencils) are ordered in bands of EQD_XI_YB rows: x3 groups
+// of a band in turn, its rows fastest, so a unit's x3 neighbours run EQD_XI_YB units later.
+#ifndef EQD_XI_YB
+#define EQD_XI_YB 32
+#endif
 
 namespace eqd {
-
-constexpr int kXiBlock = 1024;  // elements of one (line, component) block
-
-// Elements per tile vector of an x1-inner grid.
-__host__ __device__ inline size_t xi_tile_elems(int NY) { return (size_t)NY * NY * 3 * kXiBlock; }
-
-__device__ __forceinline__ size_t xi_elem(int yz, int c, int m, int j) {
-    return ((size_t)(yz * 3 + c) << 10) + ((j >> 2) << 6) + (m << 2) + (j & 3);
-}
 
 template <typename T>
 using xv4 = T __attribute__((ext_vector_type(4)));
@@ -144,12 +153,12 @@ template <typename T, int Z>
 struct XiUnit {
     const PencilView<T>& op;
     __amdgpu_buffer_rsrc_t rs;
-    int voff, NY, y, z0, nk;
+    int voff, blk, NY, y, z0, nk;  // blk: elements of a (line, component) block
     const T* Tsy;
     const T* Toy;
     T cTs[4], cTo[4], cD1[4], cDs[4];  // x2 band taps of row y at dy = -2, -1, 1, 2
     int rr[4];                         // rows y + dy, clamped
-    __device__ int line(int yy, int zz, int c) const { return ((yy + NY * zz) * 3 + c) * kXiBlock * (int)sizeof(T); }
+    __device__ int line(int yy, int zz, int c) const { return ((yy + NY * zz) * 3 + c) * blk * (int)sizeof(T); }
     __device__ xv4<T> ld(int yy, int zz, int c) const { return xload<T>(rs, voff, line(yy, zz, c)); }
 };
 
@@ -259,6 +268,16 @@ __device__ __forceinline__ void xi_plane(const XiUnit<T, Z>& U, xv4<T> (&o)[Z][3
     xi_terms_b<T, Z, P>(U, v2, v0, o, a);
 }
 
+// Unit u of a grid of NY rows and nzg x3 groups of Z pencils: its row y and first plane z0.
+template <int Z>
+__device__ __forceinline__ void xi_unit(int u, int NY, int nzg, int& y, int& z0) {
+    const int band = u / (EQD_XI_YB * nzg), r = u - band * (EQD_XI_YB * nzg);
+    const int yb = min(EQD_XI_YB, NY - band * EQD_XI_YB);  // rows of this band
+    const int zg = r / yb;
+    y = band * EQD_XI_YB + (r - zg * yb);
+    z0 = zg * Z;
+}
+
 template <typename T, int Z, int... P>
 __device__ __forceinline__ void xi_planes(const XiUnit<T, Z>& U, xv4<T> (&o)[Z][3], xv4<T> (&a)[Z][3],
                                           std::integer_sequence<int, P...>) {
@@ -289,7 +308,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(xi_waves
     __shared__ T xs[16 * WS], xo[16 * WO], xm[16 * WO];
     // per-wave transpose tiles of the matrix-core x1 bands (FP32): six inputs per pencil
     __shared__ __attribute__((aligned(16))) float xtile[sizeof(T) == 4 ? 4 * 6 * 16 * kXtRow : 1];
-    __shared__ double sm[64];
+    __shared__ double sm[4][64];
     // source form of the x1 bands: entry (m, d) = band row m - d, tap d (0 off the 16 rows)
     for (int i = threadIdx.x; i < 16 * WS; i += kBlock) {
         const int m = i / WS, d = i % WS - kRxS, r = m - d;
@@ -315,10 +334,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(xi_waves
             bMx[st] = xband_at(op.MX, kRxO, n, m);
         }
     }
-    const size_t tfl = xi_tile_elems(NY);
+    const size_t tfl = xi_tile_elems(NX, NY);
     const size_t off = (size_t)tile * tfl;
-    const int col0 = 16 * wv + 4 * cq;  // the lane's columns col0 .. col0 + 3
-    const int lo = ((4 * wv + cq) * 16 + n) * 4;  // their element offset in a (line, component) block
+#if EQD_XI_SLAB
+    const int G4 = (int)gridDim.x >> 2;
+    const int slab = g / G4, gs = g % G4;
+#else
+    const int slab = wv;
+#endif
+    const int col0 = 16 * slab + 4 * cq;  // the lane's columns col0 .. col0 + 3
+    const int lo = ((4 * slab + cq) * 16 + n) * 4;  // their element offset in a (line, component) block
     bool mine[4] = {true, true, true, true};
     if constexpr (MODE == 1) {
 #pragma unroll
@@ -326,13 +351,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(xi_waves
     }
     const T mw = (live && n == op.mface) ? T(2) : T(1);
     const T hx = live ? op.Hx[n] : T(0);
-    XiUnit<T, Z> U{op, eqd_rsrc(in + off, (uint32_t)(tfl * sizeof(T))), lo * (int)sizeof(T), NY, 0, 0, 0,
+    XiUnit<T, Z> U{op, eqd_rsrc(in + off, (uint32_t)(tfl * sizeof(T))), lo * (int)sizeof(T), 1024, NY, 0, 0, 0,
                    nullptr, nullptr, {}, {}, {}, {}, {}};
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const int nzg = (NY + Z - 1) / Z, nunits = NY * nzg;
+#if EQD_XI_SLAB
+    for (int i = 0;; ++i) {
+        const int u0 = EQD_XI_SPREAD ? gs + G4 * (wv + 4 * i) : 4 * (gs + G4 * i) + wv;
+        if (u0 >= nunits) break;
+#else
     for (int u0 = g; u0 < nunits; u0 += (int)gridDim.x) {
+#endif
         const int u = uni(u0);
-        const int y = u % NY, z0 = (u / NY) * Z;
+        int y, z0;
+        xi_unit<Z>(u, NY, nzg, y, z0);
         U.y = y;
         U.z0 = z0;
         U.nk = min(Z, NY - z0);
@@ -423,7 +455,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(xi_waves
                 xpair4<kRxO, kRxO>(xo + oo, xm + oo, p2, a[k][2], o[k][2]);
             }
             const T hw = -(hx * (hy * op.Hy[zk]));
-            const size_t e = off + xi_elem(y + NY * zk, 0, 0, 0) + lo;
+            const size_t e = off + (size_t)(y + NY * zk) * 3 * 1024 + lo;
             double d[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -432,15 +464,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(xi_waves
                     const xv4<T> pv = c == 0 ? p0 : (c == 1 ? p1 : p2);
                     // padding lanes store zeros: every 256-byte segment of q is written whole
                     __builtin_nontemporal_store(live ? v : xv4<T>{T(0), T(0), T(0), T(0)},
-                                                (xv4<T>*)(q + e + c * kXiBlock));
+                                                (xv4<T>*)(q + e + c * 1024));
 #pragma unroll
                     for (int i = 0; i < 4; ++i) d[i] += (double)pv[i] * (double)v[i];
                 } else {
                     const double dz = dinv ? dinv[(size_t)(n + NX * (y + NY * zk)) * 3 + c] : 1.0;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const double rv = b[e + c * kXiBlock + i] - (double)v[i];
-                        if (live && mine[i]) rout[e + c * kXiBlock + i] = (R)rv;
+                        const double rv = b[e + c * 1024 + i] - (double)v[i];
+                        if (live && mine[i]) rout[e + c * 1024 + i] = (R)rv;
                         d[i] += rv * (dz * rv);
                     }
                 }
@@ -457,18 +489,244 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(xi_waves
         s += __shfl_xor(s, 4, 16);
         s += __shfl_xor(s, 2, 16);
         s += __shfl_xor(s, 1, 16);
-        if (n == 0) sm[col0 + i] = s;
+        if (n == 0) sm[wv][4 * cq + i] = s;
     }
     __syncthreads();
-    if (threadIdx.x < 64) partial[((size_t)tile * gridDim.x + g) * 64 + threadIdx.x] = sm[threadIdx.x];
+    if (threadIdx.x < 64) {
+        const int j = threadIdx.x;
+#if EQD_XI_SLAB
+        // the four waves' sums in a fixed order; the other slabs' columns get 0
+        const double v = (j >> 4) == slab ? ((sm[0][j & 15] + sm[1][j & 15]) + sm[2][j & 15]) + sm[3][j & 15] : 0.0;
+#else
+        const double v = sm[j >> 4][j & 15];
+#endif
+        partial[((size_t)tile * gridDim.x + g) * 64 + j] = v;
+    }
 }
 
-// Slot geometry of the vector kernels: thread t visits 16-byte slots v = g 256 + t + k G 256 of
-// its tile, all with node m = t % 16 and column quad t / 16 (in FP32 slots).
+// ---------------------------------------------------------------------------------------------
+// Lines of 17 to 64 x1 nodes. A block's four waves are the x1 segments (nodes 16 w ..
+// 16 w + 15) of one 16-column slab and march the same unit of Z pencils. The y/z terms are those
+// of k_xi_pencil; the x1 bands need neighbours across segments, so at the end of a pencil every
+// wave copies its own lines and accumulated derivatives into an LDS line (with zero rows past
+// the ends), and each lane applies its node's band rows to the neighbours it reads there.
+constexpr int kXlHalo = 8;                   // zero rows on each side of the LDS line (>= the x1 reach)
+constexpr int kXlRows = 64 + 2 * kXlHalo;    // rows of one LDS line: up to 64 nodes + halos
+
+template <typename T, int R>
+__device__ __forceinline__ void xl_band(const T* rowc, const T* line, int m, int cq, xv4<T>& o) {
+#pragma unroll
+    for (int d = -R; d <= R; ++d) {
+        const T c = rowc[R + d];
+        const xv4<T> v = *reinterpret_cast<const xv4<T>*>(line + ((kXlHalo + m + d) * 16 + 4 * cq));
+        o += c * v;
+    }
+}
+
+template <typename T, typename R, int MODE, int Z>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 3))) void k_xl_pencil(
+    PencilView<T> op, const T* __restrict__ in, T* __restrict__ q, const double* __restrict__ b,
+    R* __restrict__ rout, const ColState* __restrict__ cs, const int* __restrict__ tile_on,
+    double* __restrict__ partial, const double* __restrict__ dinv) {
+    int tile, g;
+    tile_group(tile, g);
+#if EQD_XI_REV
+    tile = (int)gridDim.y - 1 - tile;
+#endif
+    if (!tile_on[tile]) return;
+    constexpr int WS = 2 * kRxS + 1, WO = 2 * kRxO + 1;
+    const int NX = op.NX, NY = op.NY;
+    // band rows of the 64 node slots (zero past NX), and the six LDS lines of a pencil
+    __shared__ T bS[64 * WS], bO[64 * WO], bM[64 * WO];
+    __shared__ __attribute__((aligned(16))) T xl[6 * kXlRows * 16];
+    __shared__ double sm[4][16];
+    for (int i = threadIdx.x; i < 64 * WS; i += kBlock) bS[i] = i / WS < NX ? op.AXs[i] : T(0);
+    for (int i = threadIdx.x; i < 64 * WO; i += kBlock) {
+        bO[i] = i / WO < NX ? op.AXo[i] : T(0);
+        bM[i] = i / WO < NX ? op.MX[i] : T(0);
+    }
+    for (int i = threadIdx.x; i < 6 * kXlRows * 16; i += kBlock) xl[i] = T(0);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int n = lane & 15, cq = lane >> 4;
+    const int S = (NX + 15) >> 4;        // segments of the line
+    const bool wlive = wv < S;            // the wave has nodes
+    const int m = 16 * wv + n;            // the lane's node
+    const bool live = m < NX;
+    const int mc = live ? m : NX - 1;
+    const int G4 = (int)gridDim.x >> 2;
+    const int sl = g / G4;                // the block's 16-column slab (slab-major over the tile)
+    const int qd = 4 * sl + cq;           // the lane's column quad
+    const size_t tfl = xi_tile_elems(NX, NY);
+    const size_t off = (size_t)tile * tfl;
+    const int B = xi_nxb(NX);
+    const int blk = 64 * B;
+    const int lo = qd * 4 * B + 4 * mc;  // the lane's element offset in a (line, component) block
+    const bool st = m < B;               // lanes m < B store (zeros past NX): whole 128-byte lines
+    bool mine[4] = {true, true, true, true};
+    if constexpr (MODE == 1) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) mine[i] = cs[(size_t)tile * 64 + 4 * qd + i].rel_flag != 0;
+    }
+    const T mw = (live && m == op.mface) ? T(2) : T(1);
+    const T hx = live ? op.Hx[m] : T(0);
+    XiUnit<T, Z> U{op, eqd_rsrc(in + off, (uint32_t)(tfl * sizeof(T))), lo * (int)sizeof(T), blk, NY, 0, 0, 0,
+                   nullptr, nullptr, {}, {}, {}, {}, {}};
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int nzg = (NY + Z - 1) / Z, nunits = NY * nzg;
+    for (int u0 = g % G4; u0 < nunits; u0 += G4) {
+        const int u = uni(u0);
+        int y, z0;
+        xi_unit<Z>(u, NY, nzg, y, z0);
+        U.y = y;
+        U.z0 = z0;
+        U.nk = min(Z, NY - z0);
+        U.Tsy = op.Ts + (size_t)y * (2 * kRyS + 1) + kRyS;
+        U.Toy = op.To + (size_t)y * (2 * kRyO + 1) + kRyO;
+        const T* D1y = op.D1 + (size_t)y * (2 * kRyO + 1) + kRyO;
+        const T* D1sy = op.D1s + (size_t)y * (2 * kRyO + 1) + kRyO;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int dy = j < 2 ? j - 2 : j - 1;
+            U.cTs[j] = ucoef(U.Tsy + dy);
+            U.cTo[j] = ucoef(U.Toy + dy);
+            U.cD1[j] = ucoef(D1y + dy);
+            U.cDs[j] = ucoef(D1sy + dy);
+            U.rr[j] = rclamp(y + dy, NY);
+        }
+        xv4<T> o[Z][3], a[Z][3];
+#pragma unroll
+        for (int k = 0; k < Z; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                o[k][c] = xv4<T>{T(0), T(0), T(0), T(0)};
+                a[k][c] = xv4<T>{T(0), T(0), T(0), T(0)};
+            }
+        const int nk = U.nk;
+        if (wlive) {
+            xi_planes<T, Z>(U, o, a, std::make_integer_sequence<int, Z + 2 * kRyO>{});
+            EQD_FENCE();
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int dy = j < 2 ? j - 4 : j + 1;
+                const T c = ucoef(U.Tsy + dy);
+                if (c != T(0)) {
+                    const int yy = rclamp(y + dy, NY);
+#pragma unroll
+                    for (int k = 0; k < Z; ++k)
+                        if (k < nk) o[k][1] += c * U.ld(yy, z0 + k, 1);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int zp = j < 2 ? z0 - 4 + j : z0 + Z + j;
+                if (zp < 0 || zp >= NY) continue;
+                bool any = false;
+                T ck[Z];
+#pragma unroll
+                for (int k = 0; k < Z; ++k) {
+                    const int dz = zp - (z0 + k);
+                    ck[k] = (k < nk && dz >= -kRyS && dz <= kRyS)
+                                ? ucoef(op.Ts + (size_t)(z0 + k) * (2 * kRyS + 1) + kRyS + dz)
+                                : T(0);
+                    any = any || ck[k] != T(0);
+                }
+                if (any) {
+                    const xv4<T> v = U.ld(y, zp, 2);
+#pragma unroll
+                    for (int k = 0; k < Z; ++k) o[k][2] += ck[k] * v;
+                }
+            }
+        }
+        const T hy = op.Hy[y];
+#pragma unroll
+        for (int k = 0; k < Z; ++k) {
+            if (k >= nk) continue;
+            const int zk = z0 + k;
+            xv4<T> p0{}, p1{}, p2{};
+            if (wlive) {
+                p0 = U.ld(y, zk, 0);
+                p1 = U.ld(y, zk, 1);
+                p2 = U.ld(y, zk, 2);
+                if (live) {
+                    const int r0 = (kXlHalo + m) * 16 + 4 * cq;
+                    *reinterpret_cast<xv4<T>*>(xl + 0 * kXlRows * 16 + r0) = p0;
+                    *reinterpret_cast<xv4<T>*>(xl + 1 * kXlRows * 16 + r0) = p1;
+                    *reinterpret_cast<xv4<T>*>(xl + 2 * kXlRows * 16 + r0) = p2;
+                    *reinterpret_cast<xv4<T>*>(xl + 3 * kXlRows * 16 + r0) = a[k][0];
+                    *reinterpret_cast<xv4<T>*>(xl + 4 * kXlRows * 16 + r0) = a[k][1];
+                    *reinterpret_cast<xv4<T>*>(xl + 5 * kXlRows * 16 + r0) = a[k][2];
+                }
+            }
+            __syncthreads();  // the unit's lines are complete in LDS
+            if (wlive) {
+                const int ms = live ? m : 0;
+                xl_band<T, kRxS>(bS + ms * WS, xl + 0 * kXlRows * 16, ms, cq, o[k][0]);
+                xl_band<T, kRxO>(bM + ms * WO, xl + 3 * kXlRows * 16, ms, cq, o[k][0]);
+                EQD_FENCE();
+                xl_band<T, kRxO>(bO + ms * WO, xl + 1 * kXlRows * 16, ms, cq, o[k][1]);
+                xl_band<T, kRxO>(bM + ms * WO, xl + 4 * kXlRows * 16, ms, cq, o[k][1]);
+                EQD_FENCE();
+                xl_band<T, kRxO>(bO + ms * WO, xl + 2 * kXlRows * 16, ms, cq, o[k][2]);
+                xl_band<T, kRxO>(bM + ms * WO, xl + 5 * kXlRows * 16, ms, cq, o[k][2]);
+            }
+            __syncthreads();  // every wave has read them before the next pencil's copies
+            if (!wlive) continue;
+            const T hw = -(hx * (hy * op.Hy[zk]));
+            const size_t e = off + (size_t)(y + NY * zk) * 3 * blk + lo;  // node mc
+            const int es = 4 * (m - mc);  // to the lane's own node (padding lanes m < B)
+            double d[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const xv4<T> v = hw * o[k][c];
+                if constexpr (MODE == 0) {
+                    const xv4<T> pv = c == 0 ? p0 : (c == 1 ? p1 : p2);
+                    if (st)
+                        __builtin_nontemporal_store(live ? v : xv4<T>{T(0), T(0), T(0), T(0)},
+                                                    (xv4<T>*)(q + e + es + c * blk));
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) d[i] += (double)pv[i] * (double)v[i];
+                } else {
+                    const double dz = dinv ? dinv[(size_t)(mc + NX * (y + NY * zk)) * 3 + c] : 1.0;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const double rv = b[e + c * blk + i] - (double)v[i];
+                        if (live && mine[i]) rout[e + c * blk + i] = (R)rv;
+                        d[i] += rv * (dz * rv);
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i] += (double)mw * d[i];
+        }
+    }
+    // per column: the 16 node lanes of a row (fixed butterfly), then the four segments in order
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double v = live ? acc[i] : 0.0;
+        v += __shfl_xor(v, 8, 16);
+        v += __shfl_xor(v, 4, 16);
+        v += __shfl_xor(v, 2, 16);
+        v += __shfl_xor(v, 1, 16);
+        if (n == 0) sm[wv][4 * cq + i] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int j = threadIdx.x;
+        partial[((size_t)tile * gridDim.x + g) * 64 + j] =
+            (j >> 4) == sl ? ((sm[0][j & 15] + sm[1][j & 15]) + sm[2][j & 15]) + sm[3][j & 15] : 0.0;
+    }
+}
+
+// Slot geometry of the vector kernels: thread t owns column quad t / 16 and nodes t % 16 + 16 s
+// (s < ceil(B / 16)) of every (line, component) block its workgroup visits (g, g + G, ...).
 struct XiSlot {
-    int m, cq;
+    int n, cq, B, S, blk;
 };
-__device__ __forceinline__ XiSlot xi_slot() { return XiSlot{(int)(threadIdx.x & 15), (int)(threadIdx.x >> 4)}; }
+__device__ __forceinline__ XiSlot xi_slot(int NX) {
+    const int B = xi_nxb(NX);
+    return XiSlot{(int)(threadIdx.x & 15), (int)(threadIdx.x >> 4), B, (B + 15) >> 4, 64 * B};
+}
 
 // Per-column partials of a vector kernel: the 16 node threads of a column quad (one 16-lane
 // row of a wave) in a fixed butterfly, then partial[tile][g][64].
@@ -486,6 +744,8 @@ __device__ __forceinline__ void xi_partial(double (&acc)[4], double* __restrict_
 }
 
 // r -= alpha q and the weighted partial r.r (r.(dinv r) with a Jacobi dinv), FP32 vectors.
+// Padding nodes (16-node blocks of shorter lines) take part with zeros, so every 256-byte
+// segment is written whole.
 template <bool PRE>
 __global__ __launch_bounds__(kBlock) void k_xi_update(int NX, int NY, int mface, float* __restrict__ r,
                                                       const float* __restrict__ q, const ColState* __restrict__ cs,
@@ -494,41 +754,43 @@ __global__ __launch_bounds__(kBlock) void k_xi_update(int NX, int NY, int mface,
     using V = xv4<float>;
     int tile, g;
     tile_group(tile, g);
-    const XiSlot sl = xi_slot();
+    const XiSlot sl = xi_slot(NX);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (tile_on[tile]) {
-        // padding nodes (m >= NX) take part with zeros, so every 256-byte segment is written whole
-        const bool pad = sl.m >= NX;
         float a[4];
         bool act[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const ColState& st = cs[(size_t)tile * 64 + 4 * sl.cq + j];
             act[j] = st.status == kActive;
-            a[j] = pad ? 0.0f : (float)st.alpha;
+            a[j] = (float)st.alpha;
         }
-        const double w = sl.m == mface ? 2.0 : 1.0;
-        const size_t off = xi_tile_elems(NY) * tile;
-        V* rv = reinterpret_cast<V*>(r + off);
-        const V* qv = reinterpret_cast<const V*>(q + off);
-        const size_t nv = xi_tile_elems(NY) / 4;
-        for (size_t v = (size_t)g * kBlock + threadIdx.x; v < nv; v += (size_t)gridDim.x * kBlock) {
-            V rr = __builtin_nontemporal_load(rv + v);
-            const V qq = __builtin_nontemporal_load(qv + v);
-            float dz = 1.0f;
-            if constexpr (PRE) {
-                const int bl = (int)(v >> 8), c = bl % 3, yz = bl / 3;
-                if (!pad) dz = dinv[(size_t)(sl.m + NX * yz) * 3 + c];
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if (act[j]) {
-                    const float rn = pad ? 0.0f : rr[j] - a[j] * qq[j];
-                    rr[j] = rn;
-                    acc[j] += w * ((double)rn * (double)(dz * rn));
+        const size_t off = xi_tile_elems(NX, NY) * tile + (size_t)sl.cq * 4 * sl.B;
+        const int nbl = NY * NY * 3;
+        for (int bl = g; bl < nbl; bl += (int)gridDim.x) {
+            for (int s = 0; s < sl.S; ++s) {
+                const int m = sl.n + 16 * s;
+                if (m >= sl.B) break;
+                const bool pad = m >= NX;
+                const double w = m == mface ? 2.0 : 1.0;
+                V* rp = reinterpret_cast<V*>(r + off + (size_t)bl * sl.blk + 4 * m);
+                const V* qp = reinterpret_cast<const V*>(q + off + (size_t)bl * sl.blk + 4 * m);
+                V rr = __builtin_nontemporal_load(rp);
+                const V qq = __builtin_nontemporal_load(qp);
+                float dz = 1.0f;
+                if constexpr (PRE) {
+                    if (!pad) dz = dinv[(size_t)(m + NX * (bl / 3)) * 3 + bl % 3];
                 }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (act[j]) {
+                        const float rn = pad ? 0.0f : rr[j] - a[j] * qq[j];
+                        rr[j] = rn;
+                        acc[j] += w * ((double)rn * (double)(dz * rn));
+                    }
+                }
+                __builtin_nontemporal_store(rr, rp);
             }
-            __builtin_nontemporal_store(rr, rv + v);
         }
     }
     xi_partial(acc, partial, tile, g);
@@ -543,8 +805,7 @@ __global__ __launch_bounds__(kBlock) void k_xi_pupdate(int NX, int NY, float* __
     using V = xv4<float>;
     int tile, g;
     tile_group(tile, g);
-    const XiSlot sl = xi_slot();
-    const bool pad = sl.m >= NX;  // padding nodes write zeros: whole 256-byte segments
+    const XiSlot sl = xi_slot(NX);
     float a[4], be[4];
     bool xu[4], pu[4];
     bool anyx = false, anyp = false;
@@ -559,44 +820,48 @@ __global__ __launch_bounds__(kBlock) void k_xi_pupdate(int NX, int NY, float* __
         anyp |= pu[j];
     }
     if (!anyx && !anyp) return;
-    const size_t off = xi_tile_elems(NY) * tile;
-    V* xv = reinterpret_cast<V*>(x + off);
-    V* pv = reinterpret_cast<V*>(p + off);
-    const V* rv = reinterpret_cast<const V*>(r + off);
-    const size_t nv = xi_tile_elems(NY) / 4;
-    for (size_t v = (size_t)g * kBlock + threadIdx.x; v < nv; v += (size_t)gridDim.x * kBlock) {
-        V pp = __builtin_nontemporal_load(pv + v);
-        if (anyx) {
-            V xx = __builtin_nontemporal_load(xv + v);
+    const size_t off = xi_tile_elems(NX, NY) * tile + (size_t)sl.cq * 4 * sl.B;
+    const int nbl = NY * NY * 3;
+    for (int bl = g; bl < nbl; bl += (int)gridDim.x) {
+        for (int s = 0; s < sl.S; ++s) {
+            const int m = sl.n + 16 * s;
+            if (m >= sl.B) break;
+            const bool pad = m >= NX;
+            const size_t e = off + (size_t)bl * sl.blk + 4 * m;
+            V* pv = reinterpret_cast<V*>(p + e);
+            V pp = __builtin_nontemporal_load(pv);
+            if (anyx) {
+                V* xv = reinterpret_cast<V*>(x + e);
+                V xx = __builtin_nontemporal_load(xv);
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (xu[j]) xx[j] = pad ? 0.0f : xx[j] + a[j] * pp[j];
-            __builtin_nontemporal_store(xx, xv + v);
-        }
-        if (anyp) {
-            const V rr = __builtin_nontemporal_load(rv + v);
-            float dz = 1.0f;
-            if constexpr (PRE) {
-                const int bl = (int)(v >> 8), c = bl % 3, yz = bl / 3;
-                if (!pad) dz = dinv[(size_t)(sl.m + NX * yz) * 3 + c];
+                for (int j = 0; j < 4; ++j)
+                    if (xu[j]) xx[j] = pad ? 0.0f : xx[j] + a[j] * pp[j];
+                __builtin_nontemporal_store(xx, xv);
             }
+            if (anyp) {
+                const V rr = __builtin_nontemporal_load(reinterpret_cast<const V*>(r + e));
+                float dz = 1.0f;
+                if constexpr (PRE) {
+                    if (!pad) dz = dinv[(size_t)(m + NX * (bl / 3)) * 3 + bl % 3];
+                }
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (pu[j]) pp[j] = pad ? 0.0f : (PRE ? dz * rr[j] : rr[j]) + be[j] * pp[j];
-            __builtin_nontemporal_store(pp, pv + v);
+                for (int j = 0; j < 4; ++j)
+                    if (pu[j]) pp[j] = pad ? 0.0f : (PRE ? dz * rr[j] : rr[j]) + be[j] * pp[j];
+                __builtin_nontemporal_store(pp, pv);
+            }
         }
     }
 }
 
 // Reliable update, part 1: x64 += xc, xc = 0 for the flagged columns (every column when all !=
-// 0; exactly the masked columns with a mask).
+// 0; exactly the masked columns with a mask). Padding nodes hold zeros in x and xc.
 __global__ __launch_bounds__(kBlock) void k_xi_xaccum(int NX, int NY, double* __restrict__ x, float* __restrict__ xc,
                                                       const ColState* __restrict__ cs,
                                                       const int* __restrict__ tile_on, int all,
                                                       const int* __restrict__ mask) {
     int tile, g;
     tile_group(tile, g);
-    const XiSlot sl = xi_slot();  // padding nodes hold zeros in x and xc: they take part as is
+    const XiSlot sl = xi_slot(NX);
     bool f[4];
     bool any = false;
 #pragma unroll
@@ -606,15 +871,19 @@ __global__ __launch_bounds__(kBlock) void k_xi_xaccum(int NX, int NY, double* __
         any |= f[j];
     }
     if (!any) return;
-    const size_t off = xi_tile_elems(NY) * tile;
-    const size_t nv = xi_tile_elems(NY) / 4;
-    for (size_t v = (size_t)g * kBlock + threadIdx.x; v < nv; v += (size_t)gridDim.x * kBlock) {
-        const size_t e = off + 4 * v;
+    const size_t off = xi_tile_elems(NX, NY) * tile + (size_t)sl.cq * 4 * sl.B;
+    const int nbl = NY * NY * 3;
+    for (int bl = g; bl < nbl; bl += (int)gridDim.x) {
+        for (int s = 0; s < sl.S; ++s) {
+            const int m = sl.n + 16 * s;
+            if (m >= sl.B) break;
+            const size_t e = off + (size_t)bl * sl.blk + 4 * m;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (f[j]) {
-                x[e + j] += (double)xc[e + j];
-                xc[e + j] = 0.0f;
+            for (int j = 0; j < 4; ++j) {
+                if (f[j]) {
+                    x[e + j] += (double)xc[e + j];
+                    xc[e + j] = 0.0f;
+                }
             }
         }
     }
@@ -626,18 +895,20 @@ __global__ __launch_bounds__(kBlock) void k_xi_selfdot(int NX, int NY, int mface
                                                        const double* __restrict__ dinv) {
     int tile, g;
     tile_group(tile, g);
-    const XiSlot sl = xi_slot();
+    const XiSlot sl = xi_slot(NX);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (sl.m < NX) {
-        const double w = sl.m == mface ? 2.0 : 1.0;
-        const size_t off = xi_tile_elems(NY) * tile;
-        const size_t nv = xi_tile_elems(NY) / 4;
-        for (size_t s = (size_t)g * kBlock + threadIdx.x; s < nv; s += (size_t)gridDim.x * kBlock) {
-            const int bl = (int)(s >> 8), c = bl % 3, yz = bl / 3;
-            const double dz = dinv ? dinv[(size_t)(sl.m + NX * yz) * 3 + c] : 1.0;
+    const size_t off = xi_tile_elems(NX, NY) * tile + (size_t)sl.cq * 4 * sl.B;
+    const int nbl = NY * NY * 3;
+    for (int bl = g; bl < nbl; bl += (int)gridDim.x) {
+        for (int s = 0; s < sl.S; ++s) {
+            const int m = sl.n + 16 * s;
+            if (m >= sl.B) break;
+            if (m >= NX) continue;
+            const double w = m == mface ? 2.0 : 1.0;
+            const double dz = dinv ? dinv[(size_t)(m + NX * (bl / 3)) * 3 + bl % 3] : 1.0;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const double vv = v[off + 4 * s + j];
+                const double vv = v[off + (size_t)bl * sl.blk + 4 * m + j];
                 acc[j] += w * (vv * (dz * vv));
             }
         }
@@ -654,8 +925,8 @@ __global__ __launch_bounds__(kBlock) void k_xi_rhs(RhsView rv, const double* __r
                                                    const double* __restrict__ dinv) {
     int tile, g;
     tile_group(tile, g);
-    const XiSlot sl = xi_slot();
     const int NX = rv.NX, NY = rv.NY, n23 = rv.n23;
+    const XiSlot sl = xi_slot(NX);
     bool on[4];
     bool any = false;
 #pragma unroll
@@ -665,44 +936,47 @@ __global__ __launch_bounds__(kBlock) void k_xi_rhs(RhsView rv, const double* __r
         any |= on[j];
     }
     if (!any) return;
-    const size_t off = xi_tile_elems(NY) * tile;
-    const size_t nv = xi_tile_elems(NY) / 4;
-    const int m = sl.m;
-    for (size_t s = (size_t)g * kBlock + threadIdx.x; s < nv; s += (size_t)gridDim.x * kBlock) {
-        const int bl = (int)(s >> 8), c = bl % 3, yz = bl / 3;
+    const size_t off = xi_tile_elems(NX, NY) * tile + (size_t)sl.cq * 4 * sl.B;
+    const int nbl = NY * NY * 3;
+    for (int bl = g; bl < nbl; bl += (int)gridDim.x) {
+        const int c = bl % 3, yz = bl / 3;
         const int y = yz % NY, z = yz / NY, iy = y + 1, iz = z + 1;
-        const size_t e = off + 4 * s;
+        for (int s = 0; s < sl.S; ++s) {
+            const int m = sl.n + 16 * s;
+            if (m >= sl.B) break;
+            const size_t e = off + (size_t)bl * sl.blk + 4 * m;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (!on[j]) continue;
-            const int col = tile * 64 + 4 * sl.cq + j;
-            double val = 0.0;
-            if (m < NX && col < nslots) {
-                const double* s2 = fields + (size_t)col * 2 * n23 * n23;
-                const double* s3 = s2 + (size_t)n23 * n23;
-                const double H = rv.Hx[m] * rv.Hy[y] * rv.Hy[z];
-                if (c > 0) {
-                    const double jt = rv.Jt[m];
-                    if (jt != 0.0) val = H * jt * (c == 1 ? s2 : s3)[iy + (size_t)n23 * iz];
-                } else {
-                    const double jn = rv.Jn[m];
-                    if (jn != 0.0) {
-                        double d = 0.0;
-                        const int sy = rv.D1full.start[y], wy = rv.D1full.width[y];
-                        for (int t = 0; t < wy; ++t)
-                            d += rv.D1full.c[(size_t)y * kTabW + t] * s2[(sy + t) + (size_t)n23 * iz];
-                        const int sz = rv.D1full.start[z], wz = rv.D1full.width[z];
-                        for (int t = 0; t < wz; ++t)
-                            d += rv.D1full.c[(size_t)z * kTabW + t] * s3[iy + (size_t)n23 * (sz + t)];
-                        val = H * jn * d;
+            for (int j = 0; j < 4; ++j) {
+                if (!on[j]) continue;
+                const int col = tile * 64 + 4 * sl.cq + j;
+                double val = 0.0;
+                if (m < NX && col < nslots) {
+                    const double* s2 = fields + (size_t)col * 2 * n23 * n23;
+                    const double* s3 = s2 + (size_t)n23 * n23;
+                    const double H = rv.Hx[m] * rv.Hy[y] * rv.Hy[z];
+                    if (c > 0) {
+                        const double jt = rv.Jt[m];
+                        if (jt != 0.0) val = H * jt * (c == 1 ? s2 : s3)[iy + (size_t)n23 * iz];
+                    } else {
+                        const double jn = rv.Jn[m];
+                        if (jn != 0.0) {
+                            double d = 0.0;
+                            const int sy = rv.D1full.start[y], wy = rv.D1full.width[y];
+                            for (int t = 0; t < wy; ++t)
+                                d += rv.D1full.c[(size_t)y * kTabW + t] * s2[(sy + t) + (size_t)n23 * iz];
+                            const int sz = rv.D1full.start[z], wz = rv.D1full.width[z];
+                            for (int t = 0; t < wz; ++t)
+                                d += rv.D1full.c[(size_t)z * kTabW + t] * s3[iy + (size_t)n23 * (sz + t)];
+                            val = H * jn * d;
+                        }
                     }
                 }
+                b[e + j] = val;
+                x[e + j] = 0.0;
+                r[e + j] = (float)val;
+                p[e + j] = (m < NX && dinv) ? (float)(dinv[(size_t)(m + NX * yz) * 3 + c] * val) : (float)val;
+                xc[e + j] = 0.0f;
             }
-            b[e + j] = val;
-            x[e + j] = 0.0;
-            r[e + j] = (float)val;
-            p[e + j] = (m < NX && dinv) ? (float)(dinv[(size_t)(m + NX * yz) * 3 + c] * val) : (float)val;
-            xc[e + j] = 0.0f;
         }
     }
 }

@@ -18,6 +18,7 @@ CASES = {
     "cube11": (11, 11, 1.0, 1.0, 2.0, 1.0, 0.4),
     "skew": (9, 13, 1.0, 1.2, 2.0, 1.0, 0.6),
     "ring": (9, 17, 400.0, 800.0, 32.04e9, 32.04e9, 800.0),   # l_f = L_fault
+    "long": (18, 13, 1.0, 1.2, 2.0, 1.0, 0.6),               # 33-node x1 lines (three 16-node segments)
 }
 
 
@@ -147,23 +148,25 @@ def test_batching_is_bitwise_invariant(precision, tile_k, name):
     assert np.array_equal(eqd.fault_stiffness(fe3, cols=cols), K_all[:, cols])
 
 
+@pytest.mark.parametrize("name", ["ring", "long"])
 @pytest.mark.parametrize("precond", ["none", "jacobi"])
-def test_x1_inner_tiles_match_node_major(monkeypatch, precond):
-    """64-column mixed-precision tiles on an n1 = 9 grid use the x1-inner layout and its kernels;
-    the node-major layout (EQD_STENCIL=pencil, read when the operator is created) is the other
-    implementation of the same iteration. Both K agree with the oracle to 1e-8 and with each other
-    to 1e-10 (FP32 rounding in another order, refined in FP64 to rtol 1e-10), with iteration totals
-    within a fraction of a percent."""
+def test_x1_inner_tiles_match_node_major(monkeypatch, name, precond):
+    """64-column mixed-precision tiles on grids with x1 lines of 9 to 64 nodes use the x1-inner
+    layout and its kernels (ring: 15-node lines, one 16-lane row per line; long: 33-node lines,
+    padded to 40, three 16-node segments per line); the node-major layout (EQD_STENCIL=pencil,
+    read when the operator is created) is the other implementation of the same iteration. Both K
+    agree with the oracle to 1e-8 and with each other to 1e-10 (FP32 rounding in another order,
+    refined in FP64 to rtol 1e-10), with iteration totals within a fraction of a percent."""
     monkeypatch.setenv("EQD_STENCIL", "pencil")
-    fa, o = pair("ring", precision="mixed", tile_k=64, precond=precond)
+    fa, o = pair(name, precision="mixed", tile_k=64, precond=precond)
     Ka = eqd.fault_stiffness(fa)
     ita = eqd.solver_report(fa.rs)["iterations"]
     monkeypatch.delenv("EQD_STENCIL")
-    fb, _ = pair("ring", precision="mixed", tile_k=64, precond=precond)
+    fb, _ = pair(name, precision="mixed", tile_k=64, precond=precond)
     Kb = eqd.fault_stiffness(fb)
     itb = eqd.solver_report(fb.rs)["iterations"]
     print(f"  node-major vs x1-inner: K {rel(Kb, Ka):.2e}, iterations {ita} vs {itb}", flush=True)
-    assert rel(Ka, oracle_K("ring")) < 1e-8 and rel(Kb, oracle_K("ring")) < 1e-8
+    assert rel(Ka, oracle_K(name)) < 1e-8 and rel(Kb, oracle_K(name)) < 1e-8
     assert rel(Kb, Ka) < 1e-10
     assert abs(itb - ita) <= 0.005 * ita
 
