@@ -40,10 +40,7 @@ void validate_grid(const eqd_grid_desc& g) {
     // byte offsets: the largest one (FP64, 3 components per stored node) must stay below 2^31 B.
     const int64_t NN = (int64_t)(2 * g.n1 - 3) * (g.n23 - 2) * (g.n23 - 2);
     const int64_t tile_bytes = NN * 3 * 64 * (int64_t)sizeof(double);
-    // x1-inner tiles (x1 lines of 9 to 64 nodes) pad the lines (to 16 nodes, or a multiple of 8)
-    const int NXm = 2 * g.n1 - 3;
-    const int64_t xi_bytes = (NXm >= 9 && NXm <= 64) ? (int64_t)xi_tile_elems(NXm, g.n23 - 2) * 8 : 0;
-    if (tile_bytes > (int64_t)INT32_MAX || xi_bytes > (int64_t)INT32_MAX)
+    if (tile_bytes > (int64_t)INT32_MAX)
         throw Error(EQD_ERR_GRID, "grid too large: " + std::to_string(NN) + " stored nodes make a 64-column FP64 tile of " +
                                       std::to_string(tile_bytes) + " B, above the 2^31 B a tile vector may span (at most " +
                                       std::to_string((int64_t)INT32_MAX / (3 * 64 * 8)) + " stored nodes)");
@@ -1350,8 +1347,12 @@ eqd_status eqd_operator_create(const eqd_grid_desc* g, eqd_op** out) {
         {
             const char* sv = std::getenv("EQD_STENCIL");
             // 9 <= NX keeps the FP64 bands of short lines at 16 rows (two 8-node segments), as the
-            // kernels read them; lines of more than 64 nodes keep the node-major pencils
-            op->xi = op->tb.NX >= 9 && op->tb.NX <= 64 && !(sv && std::strcmp(sv, "pencil") == 0);
+            // kernels read them; lines of more than 64 nodes keep the node-major pencils. x1-inner
+            // tiles pad the lines (to 16 nodes, or a multiple of 8): their FP64 tile vector must also
+            // stay below the 2^31 B of the buffer resource's offsets, else the grid keeps the
+            // node-major layout
+            const bool fits = (int64_t)xi_tile_elems(op->tb.NX, op->tb.NY) * 8 <= (int64_t)INT32_MAX;
+            op->xi = op->tb.NX >= 9 && op->tb.NX <= 64 && fits && !(sv && std::strcmp(sv, "pencil") == 0);
         }
         if (op->tb.NX > pencil_s<double>() && op->tb.NX <= 15) {
             op->p64w = op->p64;
