@@ -415,7 +415,7 @@ struct Engine {
                 if constexpr (std::is_same<T, float>::value) {
                     if (xi(s)) {
                         if (op.tb.NX <= 16)
-                            hipLaunchKernelGGL((k_xi_pencil<float, float, 0, EQD_XI_Z>), grid, dim3(kBlock), 0, st,
+                            hipLaunchKernelGGL((k_xi_pencil<float, float, 0, EQD_XI_Z>), grid, dim3(64 * EQD_XI_BW), 0, st,
                                                op.p32, p, q, (const double*)nullptr, (float*)nullptr, cs, on, part,
                                                (const double*)nullptr);
                         else

@@ -52,6 +52,10 @@
 #ifndef EQD_XI_YB
 #define EQD_XI_YB 32
 #endif
+// Waves per block of the FP32 stencil (slab-major order): consecutive units of one slab.
+#ifndef EQD_XI_BW
+#define EQD_XI_BW 4
+#endif
 
 namespace eqd {
 
@@ -294,7 +298,7 @@ constexpr int xi_waves() {
 //         columns, with the weighted partial r.r (r.(dinv r) with a Jacobi dinv).
 // Grid: (G, tiles) with G a multiple of 64; block g takes units g, g + G, ...
 template <typename T, typename R, int MODE, int Z>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(xi_waves<T, MODE, Z>()))) void k_xi_pencil(
+__global__ __launch_bounds__(64 * (MODE == 0 ? EQD_XI_BW : 4)) __attribute__((amdgpu_waves_per_eu(xi_waves<T, MODE, Z>()))) void k_xi_pencil(
     PencilView<T> op, const T* __restrict__ in, T* __restrict__ q, const double* __restrict__ b,
     R* __restrict__ rout, const ColState* __restrict__ cs, const int* __restrict__ tile_on,
     double* __restrict__ partial, const double* __restrict__ dinv) {
@@ -304,17 +308,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(xi_waves
     tile = (int)gridDim.y - 1 - tile;
 #endif
     if (!tile_on[tile]) return;
+    constexpr int BW = MODE == 0 ? EQD_XI_BW : 4;  // waves per block
+    static_assert(BW == 4 || EQD_XI_SLAB, "other block sizes take the slab-major order");
     constexpr int WS = 2 * kRxS + 1, WO = 2 * kRxO + 1;
     __shared__ T xs[16 * WS], xo[16 * WO], xm[16 * WO];
     // per-wave transpose tiles of the matrix-core x1 bands (FP32): six inputs per pencil
-    __shared__ __attribute__((aligned(16))) float xtile[sizeof(T) == 4 ? 4 * 6 * 16 * kXtRow : 1];
-    __shared__ double sm[4][64];
+    __shared__ __attribute__((aligned(16))) float xtile[sizeof(T) == 4 ? BW * 6 * 16 * kXtRow : 1];
+    __shared__ double sm[BW][64];
     // source form of the x1 bands: entry (m, d) = band row m - d, tap d (0 off the 16 rows)
-    for (int i = threadIdx.x; i < 16 * WS; i += kBlock) {
+    for (int i = threadIdx.x; i < 16 * WS; i += 64 * BW) {
         const int m = i / WS, d = i % WS - kRxS, r = m - d;
         xs[i] = (r >= 0 && r < 16) ? op.AXs[r * WS + kRxS + d] : T(0);
     }
-    for (int i = threadIdx.x; i < 16 * WO; i += kBlock) {
+    for (int i = threadIdx.x; i < 16 * WO; i += 64 * BW) {
         const int m = i / WO, d = i % WO - kRxO, r = m - d;
         xo[i] = (r >= 0 && r < 16) ? op.AXo[r * WO + kRxO + d] : T(0);
         xm[i] = (r >= 0 && r < 16) ? op.MX[r * WO + kRxO + d] : T(0);
@@ -357,7 +363,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(xi_waves
     const int nzg = (NY + Z - 1) / Z, nunits = NY * nzg;
 #if EQD_XI_SLAB
     for (int i = 0;; ++i) {
-        const int u0 = EQD_XI_SPREAD ? gs + G4 * (wv + 4 * i) : 4 * (gs + G4 * i) + wv;
+        const int u0 = EQD_XI_SPREAD ? gs + G4 * (wv + BW * i) : BW * (gs + G4 * i) + wv;
         if (u0 >= nunits) break;
 #else
     for (int u0 = g; u0 < nunits; u0 += (int)gridDim.x) {
@@ -496,7 +502,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(xi_waves
         const int j = threadIdx.x;
 #if EQD_XI_SLAB
         // the four waves' sums in a fixed order; the other slabs' columns get 0
-        const double v = (j >> 4) == slab ? ((sm[0][j & 15] + sm[1][j & 15]) + sm[2][j & 15]) + sm[3][j & 15] : 0.0;
+        double v = 0.0;
+        if ((j >> 4) == slab) {
+            v = sm[0][j & 15];
+#pragma unroll
+            for (int w = 1; w < BW; ++w) v += sm[w][j & 15];
+        }
 #else
         const double v = sm[j >> 4][j & 15];
 #endif
