@@ -685,7 +685,11 @@ def main():
                            "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK, 4), "traffic": measured_traffic(stencil_name),
                            "avg_launch_us": round(avg_us, 2), "launches": st_n,
-                           "bytes_per_launch": int(bytes_per_launch)}
+                           "bytes_per_launch": int(bytes_per_launch),
+                           # the same two passes (read p, write q) counted over every DOF of the
+                           # split-node system, as the per-iteration 10 x DOF x 4 B figure counts them
+                           "achieved_full_dof": round(achieved * dof / nred / 1e9, 1),
+                           "frac_full_dof": round(achieved * dof / nred / HBM_PEAK, 4)}
         out["kernel_share"] = {n: round(v[0] / tot_ms, 4) for n, v in kt.items()}
         out["survey_algorithmic_GBps"] = round(survey_gbs / 1e9, 1)
         # the survey's full formula: + 6 DOF 8 B per refinement sweep of a column (here every
