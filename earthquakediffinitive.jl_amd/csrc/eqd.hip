@@ -490,10 +490,10 @@ struct Engine {
                 if (xi(s)) {
                     if (dinv)
                         hipLaunchKernelGGL((k_xi_pupdate<true>), grid, dim3(kBlock), 0, st, op.tb.NX, op.tb.NY, xu, p,
-                                           r, cs, (const float*)dinv);
+                                           r, cs, (const float*)dinv, w.x.as<double>(), (const int*)rel);
                     else
                         hipLaunchKernelGGL((k_xi_pupdate<false>), grid, dim3(kBlock), 0, st, op.tb.NX, op.tb.NY, xu,
-                                           p, r, cs, (const float*)nullptr);
+                                           p, r, cs, (const float*)nullptr, w.x.as<double>(), (const int*)rel);
                     return;
                 }
             }
@@ -509,8 +509,7 @@ struct Engine {
         if (mixed) {
             timed(4, [&] {
                 if (xi(s))
-                    hipLaunchKernelGGL(k_xi_xaccum, grid, dim3(kBlock), 0, st, op.tb.NX, op.tb.NY, w.x.as<double>(),
-                                       w.xc.as<float>(), cs, rel, 0, (const int*)nullptr);
+                    return;  // folded into k_xi_pupdate
                 else
                     hipLaunchKernelGGL((k_xaccum<K>), grid, dim3(kBlock), 0, st, op.NN, w.x.as<double>(),
                                        w.xc.as<float>(), cs, rel, 0, (const int*)nullptr);

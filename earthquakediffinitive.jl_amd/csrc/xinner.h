@@ -807,30 +807,37 @@ __global__ __launch_bounds__(kBlock) void k_xi_update(int NX, int NY, int mface,
     xi_partial(acc, partial, tile, g);
 }
 
-// x += alpha p (columns updated this iteration) and p = r + beta p (dinv r + beta p with a
-// Jacobi dinv) for the columns still active, FP32 vectors.
+// x_c += alpha p (columns updated this iteration) and p = r + beta p (dinv r + beta p with a
+// Jacobi dinv) for the columns still active, FP32 vectors. Columns flagged for a reliable update
+// in an active tile then fold x_c into the FP64 x and clear it (k_xi_xaccum's step, the same
+// arithmetic, without another pass over x_c).
 template <bool PRE>
 __global__ __launch_bounds__(kBlock) void k_xi_pupdate(int NX, int NY, float* __restrict__ x, float* __restrict__ p,
                                                        const float* __restrict__ r, const ColState* __restrict__ cs,
-                                                       const float* __restrict__ dinv) {
+                                                       const float* __restrict__ dinv, double* __restrict__ x64,
+                                                       const int* __restrict__ tile_on) {
     using V = xv4<float>;
+    using V64 = xv4<double>;
     int tile, g;
     tile_group(tile, g);
     const XiSlot sl = xi_slot(NX);
     float a[4], be[4];
-    bool xu[4], pu[4];
-    bool anyx = false, anyp = false;
+    bool xu[4], pu[4], fo[4];
+    bool anyx = false, anyp = false, anyf = false;
+    const bool ton = x64 && tile_on[tile];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const ColState& st = cs[(size_t)tile * 64 + 4 * sl.cq + j];
         xu[j] = st.xpend != 0;
         pu[j] = st.status == kActive;
+        fo[j] = ton && st.rel_flag;
         a[j] = (float)st.alpha;
         be[j] = (float)st.beta;
         anyx |= xu[j];
         anyp |= pu[j];
+        anyf |= fo[j];
     }
-    if (!anyx && !anyp) return;
+    if (!anyx && !anyp && !anyf) return;
     const size_t off = xi_tile_elems(NX, NY) * tile + (size_t)sl.cq * 4 * sl.B;
     const int nbl = NY * NY * 3;
     for (int bl = g; bl < nbl; bl += (int)gridDim.x) {
@@ -841,12 +848,23 @@ __global__ __launch_bounds__(kBlock) void k_xi_pupdate(int NX, int NY, float* __
             const size_t e = off + (size_t)bl * sl.blk + 4 * m;
             V* pv = reinterpret_cast<V*>(p + e);
             V pp = __builtin_nontemporal_load(pv);
-            if (anyx) {
+            if (anyx || anyf) {
                 V* xv = reinterpret_cast<V*>(x + e);
                 V xx = __builtin_nontemporal_load(xv);
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                     if (xu[j]) xx[j] = pad ? 0.0f : xx[j] + a[j] * pp[j];
+                if (anyf) {
+                    V64* wv = reinterpret_cast<V64*>(x64 + e);
+                    V64 ww = *wv;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (fo[j]) {
+                            ww[j] += (double)xx[j];
+                            xx[j] = 0.0f;
+                        }
+                    *wv = ww;
+                }
                 __builtin_nontemporal_store(xx, xv);
             }
             if (anyp) {
