@@ -53,6 +53,9 @@
 #define EQD_XI_YB 32
 #endif
 // Waves per block of the FP32 stencil (slab-major order): consecutive units of one slab.
+#ifndef EQD_XI_UPD_U
+#define EQD_XI_UPD_U 2  // (line, component) blocks per step of the r update
+#endif
 #ifndef EQD_XI_BW
 #define EQD_XI_BW 4
 #endif
@@ -778,29 +781,40 @@ __global__ __launch_bounds__(kBlock) void k_xi_update(int NX, int NY, int mface,
         }
         const size_t off = xi_tile_elems(NX, NY) * tile + (size_t)sl.cq * 4 * sl.B;
         const int nbl = NY * NY * 3;
-        for (int bl = g; bl < nbl; bl += (int)gridDim.x) {
+        const int G = (int)gridDim.x;
+        // EQD_XI_UPD_U (line, component) blocks per step, all loaded before any is stored
+        for (int bl0 = g; bl0 < nbl; bl0 += EQD_XI_UPD_U * G) {
             for (int s = 0; s < sl.S; ++s) {
                 const int m = sl.n + 16 * s;
                 if (m >= sl.B) break;
                 const bool pad = m >= NX;
                 const double w = m == mface ? 2.0 : 1.0;
-                V* rp = reinterpret_cast<V*>(r + off + (size_t)bl * sl.blk + 4 * m);
-                const V* qp = reinterpret_cast<const V*>(q + off + (size_t)bl * sl.blk + 4 * m);
-                V rr = __builtin_nontemporal_load(rp);
-                const V qq = __builtin_nontemporal_load(qp);
-                float dz = 1.0f;
-                if constexpr (PRE) {
-                    if (!pad) dz = dinv[(size_t)(m + NX * (bl / 3)) * 3 + bl % 3];
+                V rr[EQD_XI_UPD_U], qq[EQD_XI_UPD_U];
+#pragma unroll
+                for (int u = 0; u < EQD_XI_UPD_U; ++u) {
+                    const int bl = bl0 + u * G;
+                    if (bl >= nbl) continue;
+                    rr[u] = __builtin_nontemporal_load(reinterpret_cast<const V*>(r + off + (size_t)bl * sl.blk + 4 * m));
+                    qq[u] = __builtin_nontemporal_load(reinterpret_cast<const V*>(q + off + (size_t)bl * sl.blk + 4 * m));
                 }
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    if (act[j]) {
-                        const float rn = pad ? 0.0f : rr[j] - a[j] * qq[j];
-                        rr[j] = rn;
-                        acc[j] += w * ((double)rn * (double)(dz * rn));
+                for (int u = 0; u < EQD_XI_UPD_U; ++u) {
+                    const int bl = bl0 + u * G;
+                    if (bl >= nbl) continue;
+                    float dz = 1.0f;
+                    if constexpr (PRE) {
+                        if (!pad) dz = dinv[(size_t)(m + NX * (bl / 3)) * 3 + bl % 3];
                     }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (act[j]) {
+                            const float rn = pad ? 0.0f : rr[u][j] - a[j] * qq[u][j];
+                            rr[u][j] = rn;
+                            acc[j] += w * ((double)rn * (double)(dz * rn));
+                        }
+                    }
+                    __builtin_nontemporal_store(rr[u], reinterpret_cast<V*>(r + off + (size_t)bl * sl.blk + 4 * m));
                 }
-                __builtin_nontemporal_store(rr, rp);
             }
         }
     }
