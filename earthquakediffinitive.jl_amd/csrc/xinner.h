@@ -9,12 +9,15 @@
 // A 16-byte access holds four columns of one node, so 16 lanes with consecutive nodes read
 // 256 contiguous bytes. The FP64 vectors b and x use the same element order.
 //
-// Stencil. Lanes are (x1 node n = lane % 16, column quad lane / 16); wave w of a block takes the
-// columns 16 w .. 16 w + 15, so one 16-byte load per lane (32 bytes in FP64) fetches the line
-// for the wave's columns. The x1 operators act across the 16 lanes of a row through DPP row
-// shifts, each lane holding the band row of its own node. A block marches the x3 planes of a
-// unit of Z pencils (x2 row y, planes z0 .. z0 + Z - 1): a line loaded once serves every pencil
-// of the unit that needs it, and the mixed x2-x3 term is sum-factorised per plane.
+// Stencil. Lanes are (x1 node n = lane % 16, column quad lane / 16): a wave holds 16 columns (one
+// slab) of 16 nodes, so one 16-byte load per lane (32 bytes in FP64) fetches a line for the
+// wave's columns. A wave marches the x3 planes of a unit of Z pencils (x2 row y, planes z0 ..
+// z0 + Z - 1): a line loaded once serves every pencil of the unit that needs it, and the mixed
+// x2-x3 term is sum-factorised per plane. The x1 operators act across the 16 lanes of a row: on
+// the matrix cores in FP32, through DPP row shifts in FP64 (each lane holding the band row of
+// its own node). The four waves of a block take four consecutive units of one slab, and the
+// groups of a tile take its slabs in turn (slab-major), so the lines in flight on an XCD stay
+// within one slab of one tile.
 //
 // Lines of more than 16 nodes (k_xl_pencil): the four waves of a block are the x1 segments of
 // 16 nodes of one 16-column slab, and the x1 bands read their neighbours across segments from
