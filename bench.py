@@ -381,32 +381,46 @@ def bench_ensemble(args):
               f"steps {min(accp)}-{max(accp)}", file=sys.stderr, flush=True)
         del epw, mpw
     # BP8-QD-PW over the full 30 days with the implicit-local integrator (per-node 3 x 3 Newton
-    # blocks inside L-stable SDIRK4 stages), one trajectory per member: the first members of the
-    # same seeded set, each a build_model with its own Q0, a and sigma0 (Toeplitz K s)
+    # blocks inside L-stable SDIRK4 stages): the same seeded members in lockstep Newton iterations
+    # (one batched FFT K S and one rhs! over all members per iteration, each member with its own
+    # step size and error control) on a Peaceman model with the Toeplitz K; member 0 also alone
+    # (eqd_bp8_integrate_imex) for the sequential rate and a trajectory check
     pw_imex = None
     if args.ens_pw_imex_members > 0:
-        runs = []
+        ne = min(args.ens_pw_imex_members, args.ens_members)
+        mpi = bp8.build_model(par=par, dz=20.0, L_fault=800.0, L_normal=400.0, injection="peaceman",
+                              stiffness="toeplitz", precision="mixed")
+        ei = Ensemble(mpi, ne, **{k: v[:ne] for k, v in mem.items()})
+        print(f"bench: ensemble PW imex {ne} members, {args.ens_pw_imex_hours} h ...", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        for j in range(args.ens_pw_imex_members):
-            pj = bp8.benchmark_parameters(l_f=800.0, dz=20.0, Q0=float(mem["Q0"][j]), a=float(mem["a"][j]),
-                                          sigma0=float(mem["sigma0"][j]))
-            mj = bp8.build_model(par=pj, dz=20.0, L_fault=800.0, L_normal=400.0, injection="peaceman",
-                                 stiffness="toeplitz", precision="mixed")
-            sol = bp8.run_bp8(mj, tspan=(0.0, args.ens_pw_imex_hours * 3600.0), saveat=3600.0, alg="imex",
-                              maxiters=400000)
-            runs.append(dict(member=j, accepted_steps=int(sol.naccept), rejected=int(sol.nreject), rhs=int(sol.nrhs),
-                             device_ms=round(sol.ms, 1), retcode=sol.retcode,
-                             p_well_max_MPa=round(float(sol.u[:, -1].max()) / 1e6, 3)))
-            print(f"bench: ensemble PW imex member {j}: {sol.naccept} steps, {sol.ms / 1e3:.2f} s", file=sys.stderr,
-                  flush=True)
-            del mj
-        dev_s = sum(r["device_ms"] for r in runs) / 1e3
-        pw_imex = {"hours": args.ens_pw_imex_hours, "members_run": len(runs), "runs": runs,
-                   "member_steps_per_s": round(sum(r["accepted_steps"] for r in runs) / dev_s, 1),
-                   "member_hours_per_s": round(len(runs) * args.ens_pw_imex_hours / dev_s, 1),
-                   "wall_s": round(time.perf_counter() - t0, 2),
-                   "note": "one trajectory per member (eqd_bp8_integrate_imex); Tsit5 needs ~1e6 steps per 100 h at "
-                           "dz = 20 m on this variant"}
+        sti = ei.run(tspan=(0.0, args.ens_pw_imex_hours * 3600.0), saveat=3600.0, alg="imex", maxiters=400000)
+        wall = time.perf_counter() - t0
+        acci = [s.naccept for s in sti]
+        s0 = ei.solution(0)
+        p0 = bp8.benchmark_parameters(l_f=800.0, dz=20.0, Q0=float(mem["Q0"][0]), a=float(mem["a"][0]),
+                                      sigma0=float(mem["sigma0"][0]))
+        m0 = bp8.build_model(par=p0, dz=20.0, L_fault=800.0, L_normal=400.0, injection="peaceman",
+                             stiffness="toeplitz", precision="mixed")
+        r0 = bp8.run_bp8(m0, tspan=(0.0, args.ens_pw_imex_hours * 3600.0), saveat=3600.0, alg="imex",
+                         maxiters=400000)
+        nrow = min(len(s0.t), len(r0.t))
+        pwell_diff = float(np.max(np.abs(s0.u[:nrow, -1] - r0.u[:nrow, -1]) / np.abs(r0.u[:nrow, -1]).max()))
+        pw_imex = {"hours": args.ens_pw_imex_hours, "members": ne, "wall_s": round(wall, 2),
+                   "device_loop_ms": round(ei.loop_ms, 1),
+                   "accepted_steps_min_max": [int(min(acci)), int(max(acci))],
+                   "member_steps_per_s": round(sum(acci) / (ei.loop_ms * 1e-3), 1),
+                   "member_hours_per_s": round(ne * args.ens_pw_imex_hours / (ei.loop_ms * 1e-3), 1),
+                   "retcodes_ok": bool(all(s.retcode == 0 for s in sti)),
+                   "single_member0": {"device_ms": round(r0.ms, 1), "accepted_steps": int(r0.naccept),
+                                      "p_well_max_MPa": round(float(r0.u[:, -1].max()) / 1e6, 3),
+                                      "ensemble_member0_steps": int(sti[0].naccept),
+                                      "p_well_max_rel_diff": pwell_diff},
+                   "speedup_vs_sequential_single": round(ne * r0.ms / ei.loop_ms, 1),
+                   "note": "eqd_bp8_ensemble_integrate_imex; Tsit5 needs ~1e6 steps per 100 h at dz = 20 m on "
+                           "this variant"}
+        print(f"bench: ensemble PW imex {ne} members {args.ens_pw_imex_hours} h: loop {ei.loop_ms / 1e3:.2f} s, "
+              f"steps {min(acci)}-{max(acci)}; member 0 alone {r0.ms / 1e3:.2f} s", file=sys.stderr, flush=True)
+        del ei, mpi, m0
     res = {"nf": m.nf, "members": args.ens_members, "hours": args.ens_hours, "K_build_ms": round(t_build * 1e3, 1),
            "run_ms": round(t_run * 1e3, 1), "device_loop_ms": round(ens_loop_ms, 1),
            "accepted_steps_min_max": [int(min(acc)), int(max(acc))],
@@ -561,9 +575,9 @@ def main():
     ap.add_argument("--ens_hours", type=float, default=720.0, help="ensemble horizon (30 days)")
     ap.add_argument("--ens_pw_hours", type=float, default=3.0,
                     help="horizon of the Peaceman-well ensemble (0: skip; its steps grow fast with the horizon)")
-    ap.add_argument("--ens_pw_imex_members", type=int, default=4,
+    ap.add_argument("--ens_pw_imex_members", type=int, default=256,
                     help="members of the Peaceman-well ensemble run over --ens_pw_imex_hours with the implicit-local "
-                         "integrator, one trajectory each")
+                         "integrator (lockstep over the members; at most --ens_members)")
     ap.add_argument("--ens_pw_imex_hours", type=float, default=720.0)
     ap.add_argument("--ens_pw_maxiters", type=int, default=3000,
                     help="step budget per Peaceman-well member (a longer run returns MaxIters)")

@@ -861,6 +861,7 @@ struct EnsDev {
     int64_t nsave_max;
     const double* D;       // splits x 2nf x M: partial K y of the current stage
     int splits;
+    double* tv;            // null, or 4 x nf x M: T2, T3, V2, V3 of the last evaluation (implicit-local)
 };
 
 // One rhs! per (node, member): thread i over nodes, member m = blockIdx.y. d.D holds the
@@ -914,6 +915,13 @@ __global__ __launch_bounds__(256) void k_ens_stage(EnsDev d, int stage, const do
             d.Vprev[nm] = fmax(vm, 1e-30);
         }
         d.Vm[nm] = vm;
+        if (d.tv) {
+            const size_t zs = (size_t)c.nf * d.M;
+            d.tv[nm] = T2;
+            d.tv[zs + nm] = T3;
+            d.tv[2 * zs + nm] = v2;
+            d.tv[3 * zs + nm] = v3;
+        }
         double f[5];
         f[0] = v2;
         f[1] = v3;
@@ -1265,6 +1273,188 @@ __global__ void k_imex_err(Dev d, const double* __restrict__ un, const double* _
     out[e] = r * r;
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// The implicit-local integrator over an ensemble: the members advance in lockstep steps, each with
+// its own step size, stages and Newton iterations of k_imex_newton's form (per (node, member)),
+// every rhs! and K S over all members at once. Member m's entries sit at m ldU + e; a member
+// takes part in a launch when act[m] is set.
+
+// Stage start per member: base = u_n + h sum_{j < stage} A[stage][j] F_j, U = base + h gamma Fg.
+__global__ void k_eimex_base(int ns, int ld, int stage, const double* __restrict__ h, const int* __restrict__ act,
+                             const double* __restrict__ un, const double* __restrict__ F, int64_t fst,
+                             const double* __restrict__ Fg, double* __restrict__ base, double* __restrict__ U) {
+    const int m = blockIdx.y, e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= ns || !act[m]) return;
+    const size_t o = (size_t)m * ld + e;
+    double acc = 0.0;
+    for (int j = 0; j < stage; ++j) acc += kSdA[stage][j] * F[(size_t)j * fst + o];
+    const double b = un[o] + h[m] * acc;
+    base[o] = b;
+    U[o] = b + h[m] * 0.25 * Fg[o];
+}
+
+// One Newton iteration per (node, member) on U = base + hg f(U), f(U) in F (just evaluated, its
+// tractions and slip rates in d.tv); partial[m nf + i] = the node's sum of (dU / scale)^2.
+__global__ void k_eimex_newton(EnsDev d, const double* __restrict__ Kd, const double* __restrict__ F,
+                               const double* __restrict__ base, const double* __restrict__ hgv,
+                               const double* __restrict__ rtol, const int* __restrict__ act,
+                               const double* __restrict__ un, double* __restrict__ U, double* __restrict__ partial) {
+    const Consts& c = d.c;
+    const int m = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+    const int nf = c.nf;
+    if (i >= nf) return;
+    const size_t nm = (size_t)m * nf + i;
+    if (!act[m]) {
+        partial[nm] = 0.0;
+        return;
+    }
+    const double hg = hgv[m], reltol = rtol[m], a = d.pa[m];
+    const size_t um = (size_t)m * d.ldU;
+    const size_t e0 = um + i, e1 = um + nf + i, e2 = um + 2 * nf + i, e3 = um + 3 * nf + i;
+    const double G0 = U[e0] - base[e0] - hg * F[e0];
+    const double G1 = U[e1] - base[e1] - hg * F[e1];
+    const double G2 = U[e2] - base[e2] - hg * F[e2];
+    const double G3 = U[e3] - base[e3] - hg * F[e3];
+    const double phi = U[e2];
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, -exp(-phi)}};
+    const size_t zs = (size_t)nf * d.M;
+    const double V = d.Vm[nm];
+    const double T2 = d.tv[nm], T3 = d.tv[zs + nm];
+    const double T = sqrt(T2 * T2 + T3 * T3);
+    if (d.active[i] && T > 0.0) {
+        const double sig = fmax(d.psig[m] - U[e3], c.sigma_min);
+        const double C = exp((c.fs + c.b * (log(c.Vs / c.Dc) + phi)) / a);
+        const double u = V * C / (2 * c.Vs);
+        const double r = sqrt(1 + u * u);
+        const double D = c.eta + sig * a * (C / (2 * c.Vs)) / r;
+        const double dVdT = 1.0 / D, dVdphi = -(sig * c.b * u / r) / D;
+        const double n2 = T2 / T, n3 = T3 / T;
+        const double K22 = Kd[4 * i], K23 = Kd[4 * i + 1], K32 = Kd[4 * i + 2], K33 = Kd[4 * i + 3];
+        const double dT2 = n2 * K22 + n3 * K32, dT3 = n2 * K23 + n3 * K33;
+        J[0][0] = dVdT * dT2 * n2 + V * (K22 - n2 * dT2) / T;
+        J[0][1] = dVdT * dT3 * n2 + V * (K23 - n2 * dT3) / T;
+        J[1][0] = dVdT * dT2 * n3 + V * (K32 - n3 * dT2) / T;
+        J[1][1] = dVdT * dT3 * n3 + V * (K33 - n3 * dT3) / T;
+        J[0][2] = dVdphi * n2;
+        J[1][2] = dVdphi * n3;
+        J[2][0] = -dVdT * dT2 / c.D_RS;
+        J[2][1] = -dVdT * dT3 / c.D_RS;
+        J[2][2] -= dVdphi / c.D_RS;
+    }
+    double Mx[3][4];
+    const double Gv[3] = {G0, G1, G2};
+    for (int r = 0; r < 3; ++r) {
+        for (int q = 0; q < 3; ++q) Mx[r][q] = (r == q ? 1.0 : 0.0) - hg * J[r][q];
+        Mx[r][3] = -Gv[r];
+    }
+    for (int col = 0; col < 3; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 3; ++r)
+            if (fabs(Mx[r][col]) > fabs(Mx[piv][col])) piv = r;
+        if (piv != col)
+            for (int q = 0; q < 4; ++q) {
+                const double t = Mx[col][q];
+                Mx[col][q] = Mx[piv][q];
+                Mx[piv][q] = t;
+            }
+        for (int r = col + 1; r < 3; ++r) {
+            const double f = Mx[r][col] / Mx[col][col];
+            for (int q = col; q < 4; ++q) Mx[r][q] -= f * Mx[col][q];
+        }
+    }
+    double x[3];
+    for (int r = 2; r >= 0; --r) {
+        double acc = Mx[r][3];
+        for (int q = r + 1; q < 3; ++q) acc -= Mx[r][q] * x[q];
+        x[r] = acc / Mx[r][r];
+    }
+    const double dU[4] = {x[0], x[1], x[2], -G3};
+    const size_t es[4] = {e0, e1, e2, e3};
+    const int el[4] = {i, nf + i, 2 * nf + i, 3 * nf + i};
+    double tot = 0.0;
+    for (int q = 0; q < 4; ++q) {
+        const size_t e = es[q];
+        U[e] += dU[q];
+        const double sc = abstol_of(c, el[q]) + reltol * fabs(un[e]);
+        const double r = dU[q] / sc;
+        tot += r * r;
+    }
+    if (c.injection == 1 && i == c.well_cell) {
+        const size_t e = um + 4 * nf;
+        const double G = U[e] - base[e] - hg * F[e];
+        U[e] -= G;
+        const double sc = abstol_of(c, 4 * nf) + reltol * fabs(un[e]);
+        tot += (G / sc) * (G / sc);
+    }
+    partial[nm] = tot;
+}
+
+// One block per member: out[2m] = sum of v[m n .. m n + n) (the order of k_imex_sum), out[2m+1] =
+// the member's slip-rate failure flag, which is cleared.
+__global__ __launch_bounds__(256) void k_eimex_sum(const double* __restrict__ v, int n, int* __restrict__ fail,
+                                                    double* __restrict__ out) {
+    __shared__ double red[256];
+    const int m = blockIdx.x;
+    const double* vm = v + (size_t)m * n;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) acc += vm[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * m] = red[0];
+        out[2 * m + 1] = (double)fail[m];
+        fail[m] = 0;
+    }
+}
+
+// Converged stage derivative F = (U - base) / hg.
+__global__ void k_eimex_stage_f(int ns, int ld, const double* __restrict__ U, const double* __restrict__ base,
+                                const double* __restrict__ hgv, const int* __restrict__ act, double* __restrict__ Fs) {
+    const int m = blockIdx.y, e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= ns || !act[m]) return;
+    const size_t o = (size_t)m * ld + e;
+    Fs[o] = (U[o] - base[o]) / hgv[m];
+}
+
+// Squared scaled error per entry (packed out[m ns + e]), as k_imex_err.
+__global__ void k_eimex_err(Consts c, int ld, const double* __restrict__ un, const double* __restrict__ U,
+                            const double* __restrict__ F, int64_t fst, const double* __restrict__ h,
+                            const double* __restrict__ rtol, const int* __restrict__ act, double* __restrict__ out) {
+    const int ns = c.nstate;
+    const int m = blockIdx.y, e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= ns) return;
+    if (!act[m]) {
+        out[(size_t)m * ns + e] = 0.0;
+        return;
+    }
+    const size_t o = (size_t)m * ld + e;
+    double acc = 0.0;
+    for (int j = 0; j < 5; ++j) acc += kSdBh[j] * F[(size_t)j * fst + o];
+    const double est = U[o] - (un[o] + h[m] * acc);
+    const double sc = abstol_of(c, e) + rtol[m] * fmax(fabs(un[o]), fabs(U[o]));
+    const double r = est / sc;
+    out[(size_t)m * ns + e] = r * r;
+}
+
+// Accepted members: u_n = U, f(u_n) = F_4 (stiffly accurate), and the save slot slot[m] >= 0
+// (saves: member m's rows at m save_cap ld).
+__global__ void k_eimex_commit(int ns, int ld, const int* __restrict__ acc, const long long* __restrict__ slot,
+                               int64_t save_cap, const double* __restrict__ U, const double* __restrict__ F4,
+                               double* __restrict__ un, double* __restrict__ Fn, double* __restrict__ saves) {
+    const int m = blockIdx.y, e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= ns || !acc[m]) return;
+    const size_t o = (size_t)m * ld + e;
+    const double v = U[o];
+    un[o] = v;
+    Fn[o] = F4[o];
+    if (slot[m] >= 0) saves[((size_t)m * save_cap + (size_t)slot[m]) * ld + e] = v;
+}
+
 }  // namespace bp8
 }  // namespace eqd
 
@@ -1334,6 +1524,10 @@ struct eqd_bp8_ensemble {
     int M = 0, ldU = 0;
     std::vector<double> ha, hsig, hQ0;
     DevMem Kc, pa, psig, pQ0, psrc, Vprev, Vm, fail, k, uprev, unew, ya, yb, D, partial, ctl, saves, tfix;
+    // the implicit-local integrator: tractions and slip rates of the last evaluation, per-entry
+    // partials, per-member step sizes, tolerances, masks, save slots and reduction results
+    DevMem itv, ipart, ih, ihg, irt, iact, iacc, islot, iout;
+    double* h_iout = nullptr;
     int64_t save_cap = 0, nsaved = 0;
     hipStream_t st = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1346,6 +1540,7 @@ struct eqd_bp8_ensemble {
     ~eqd_bp8_ensemble() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (h_ctl) (void)hipHostFree(h_ctl);
+        if (h_iout) (void)hipHostFree(h_iout);
         if (st) (void)hipStreamDestroy(st);
     }
     int node_blocks() const { return (base->c.nf + 255) / 256; }
@@ -1373,6 +1568,7 @@ struct eqd_bp8_ensemble {
         d.nsave_max = save_cap;
         d.D = D.as<double>();
         d.splits = splits;
+        d.tv = nullptr;
         return d;
     }
 };
@@ -1905,27 +2101,11 @@ void ens_initial_state(const eqd_bp8_ensemble& E, int m, double* u) {
     for (int i = 0; i < B.c.nf; ++i) u[2 * B.c.nf + i] = std::log(theta0);
 }
 
-void ens_integrate(eqd_bp8_ensemble& E, double t0, double tf, double saveat, double reltol_all,
-                   const double* reltols, int64_t maxiters, int64_t nsave_max, eqd_bp8_stats* stats,
-                   double* loop_ms) {
-    if (!(tf > t0) || !(saveat > 0) || nsave_max < 1)
-        throw Error(EQD_ERR_ARG, "need tf > t0, saveat > 0 and nsave_max >= 1");
-    std::vector<double> rtol(E.M, reltol_all);
-    if (reltols) std::copy(reltols, reltols + E.M, rtol.begin());
-    for (double r : rtol)
-        if (!(r > 0)) throw Error(EQD_ERR_ARG, "reltol must be positive");
-    if (maxiters <= 0) maxiters = 100000;
+// Every member's u0 in E.uprev and f(u0) in E.k (stage 0), and its first step by Hairer's rule
+// (as start_state does for one trajectory).
+std::vector<double> ens_start(eqd_bp8_ensemble& E, double t0, double tf, const std::vector<double>& rtol) {
     const eqd_bp8& B = *E.base;
     const int M = E.M, ns = B.c.nstate, ld = E.ldU, nf = B.c.nf;
-    if (nsave_max > E.save_cap) {
-        E.saves.alloc((size_t)M * nsave_max * ld * 8);
-        E.save_cap = nsave_max;
-        if (E.exec) {
-            HIPCHK(hipGraphExecDestroy(E.exec));
-            E.exec = nullptr;
-        }
-    }
-    E.nsaved = 0;
     std::vector<double> u0((size_t)ld * M, 0.0), f0((size_t)ld * M), f1((size_t)ld * M), u1((size_t)ld * M, 0.0);
     for (int m = 0; m < M; ++m) ens_initial_state(E, m, u0.data() + (size_t)m * ld);
     std::vector<double> tf0(M, t0);
@@ -1957,7 +2137,7 @@ void ens_integrate(eqd_bp8_ensemble& E, double t0, double tf, double saveat, dou
     ens_rhs(E, E.ya.as<double>(), ld, E.yb.as<double>());
     HIPCHK(hipMemcpyAsync(f1.data(), E.yb.p, f1.size() * 8, hipMemcpyDeviceToHost, E.st));
     ens_check_newton(E);
-    std::vector<Ctl> hc(M);
+    std::vector<double> dts(M);
     for (int m = 0; m < M; ++m) {
         const double reltol = rtol[m];
         const double* u = u0.data() + (size_t)m * ld;
@@ -1972,7 +2152,37 @@ void ens_integrate(eqd_bp8_ensemble& E, double t0, double tf, double saveat, dou
         const double d1 = std::sqrt(s1 / ns), d2 = std::sqrt(s2 / ns) / dt0[m];
         const double md = std::max(d1, d2);
         const double dt1 = md <= 1e-15 ? std::max(1e-6, dt0[m] * 1e-3) : std::pow(10.0, -(2 + std::log10(md)) / 5.0);
-        const double dt = std::min(std::min(100 * dt0[m], dt1), tf - t0);
+        dts[m] = std::min(std::min(100 * dt0[m], dt1), tf - t0);
+    }
+    return dts;
+}
+
+void ens_integrate(eqd_bp8_ensemble& E, double t0, double tf, double saveat, double reltol_all,
+                   const double* reltols, int64_t maxiters, int64_t nsave_max, eqd_bp8_stats* stats,
+                   double* loop_ms) {
+    if (!(tf > t0) || !(saveat > 0) || nsave_max < 1)
+        throw Error(EQD_ERR_ARG, "need tf > t0, saveat > 0 and nsave_max >= 1");
+    std::vector<double> rtol(E.M, reltol_all);
+    if (reltols) std::copy(reltols, reltols + E.M, rtol.begin());
+    for (double r : rtol)
+        if (!(r > 0)) throw Error(EQD_ERR_ARG, "reltol must be positive");
+    if (maxiters <= 0) maxiters = 100000;
+    const eqd_bp8& B = *E.base;
+    const int M = E.M, ns = B.c.nstate, ld = E.ldU, nf = B.c.nf;
+    if (nsave_max > E.save_cap) {
+        E.saves.alloc((size_t)M * nsave_max * ld * 8);
+        E.save_cap = nsave_max;
+        if (E.exec) {
+            HIPCHK(hipGraphExecDestroy(E.exec));
+            E.exec = nullptr;
+        }
+    }
+    E.nsaved = 0;
+    const std::vector<double> dts = ens_start(E, t0, tf, rtol);
+    std::vector<Ctl> hc(M);
+    for (int m = 0; m < M; ++m) {
+        const double reltol = rtol[m];
+        const double dt = dts[m];
         Ctl& c = hc[m];
         std::memset(&c, 0, sizeof(Ctl));
         c.t = t0;
@@ -2058,6 +2268,243 @@ void ens_integrate(eqd_bp8_ensemble& E, double t0, double tf, double saveat, dou
         HIPCHK(hipMemset(E.fail.p, 0, M * sizeof(int)));
         throw Error(EQD_ERR_NEWTON, "solve_slip_rate failed to converge (member " + std::to_string(failed) + ")");
     }
+}
+
+
+// The implicit-local integrator over the members (the arithmetic of integrate_imex per member):
+// every member keeps its own step size, Newton iterations, error control, saves and counters, and
+// the host drives the lockstep loop: per Newton iteration one K S and one rhs! over all members,
+// the per-(node, member) block Newton of the members still iterating, and one read of their
+// update norms. A member whose stage Newton has converged waits for the others (its U no longer
+// moves); a member that has reached tf, or failed, leaves the launches.
+void ens_integrate_imex(eqd_bp8_ensemble& E, double t0, double tf, double saveat, double reltol_all,
+                        const double* reltols, int64_t maxiters, int64_t nsave_max, eqd_bp8_stats* stats,
+                        double* loop_ms) {
+    if (!(tf > t0) || !(saveat > 0) || nsave_max < 1)
+        throw Error(EQD_ERR_ARG, "need tf > t0, saveat > 0 and nsave_max >= 1");
+    std::vector<double> rtol(E.M, reltol_all);
+    if (reltols) std::copy(reltols, reltols + E.M, rtol.begin());
+    for (double r : rtol)
+        if (!(r > 0)) throw Error(EQD_ERR_ARG, "reltol must be positive");
+    if (maxiters <= 0) maxiters = 100000;
+    eqd_bp8& B = *E.base;
+    const int M = E.M, ns = B.c.nstate, ld = E.ldU, nf = B.c.nf;
+    if (nsave_max > E.save_cap) {
+        E.saves.alloc((size_t)M * nsave_max * ld * 8);
+        E.save_cap = nsave_max;
+        if (E.exec) {
+            HIPCHK(hipGraphExecDestroy(E.exec));
+            E.exec = nullptr;
+        }
+    }
+    if (!E.itv.p) {
+        E.itv.alloc((size_t)4 * nf * M * 8);
+        E.ipart.alloc((size_t)std::max(ns, nf) * M * 8);
+        E.ih.alloc((size_t)M * 8);
+        E.ihg.alloc((size_t)M * 8);
+        E.irt.alloc((size_t)M * 8);
+        E.iact.alloc((size_t)M * sizeof(int));
+        E.iacc.alloc((size_t)M * sizeof(int));
+        E.islot.alloc((size_t)M * sizeof(long long));
+        E.iout.alloc((size_t)2 * M * 8);
+        HIPCHK(hipHostMalloc((void**)&E.h_iout, (size_t)2 * M * 8, hipHostMallocDefault));
+    }
+    if (!B.Kd.p) {
+        B.Kd.alloc((size_t)4 * nf * 8);
+        B.imex_out.alloc(2 * 8);
+        hipLaunchKernelGGL(k_imex_kdiag, dim3((nf + 255) / 256), dim3(256), 0, E.st, B.dev(), B.Kd.as<double>());
+        HIPCHK(hipGetLastError());
+    }
+    E.nsaved = 0;
+    const std::vector<double> dts = ens_start(E, t0, tf, rtol);  // u_n in uprev, f(u_n) in k[0]
+    HIPCHK(hipMemcpyAsync(E.irt.p, rtol.data(), M * 8, hipMemcpyHostToDevice, E.st));
+    EnsDev d = E.dev();
+    d.tv = E.itv.as<double>();
+    const size_t fst = (size_t)ld * M;
+    double* F = E.k.as<double>();  // stage derivatives F_0 .. F_4 in k[0..4]
+    double* Fn = F + 5 * fst;      // f(u_n)
+    double* Ft = F + 6 * fst;      // f(U) of the current Newton iterate
+    double* U = E.ya.as<double>();
+    double* base = E.yb.as<double>();
+    double* un = E.uprev.as<double>();
+    HIPCHK(hipMemcpyAsync(Fn, F, fst * 8, hipMemcpyDeviceToDevice, E.st));
+    HIPCHK(hipMemcpy2DAsync(E.saves.p, (size_t)E.save_cap * ld * 8, un, (size_t)ld * 8, (size_t)ns * 8, M,
+                            hipMemcpyDeviceToDevice, E.st));
+    std::vector<double> t(M, t0), dt(dts), next_save(M, t0 + saveat), h(M, 0.0), hg(M, 0.0), stop(M, tf), tfix(M);
+    std::vector<int64_t> nsaved(M, 1), naccept(M, 0), nreject(M, 0), nrhs(M, 2), iters(M, 0);
+    std::vector<int> retcode(M, 0), done(M, 0), act(M, 0), okv(M, 0), clipped(M, 0), nset(M, 0), conv(M, 0),
+        accv(M, 0), end_save(M, 0);
+    std::vector<long long> slot(M, -1);
+    const double sc_t = 1e-9 * saveat;
+    static const double cs[5] = {0.25, 0.75, 11.0 / 20.0, 0.5, 1.0};
+    const dim3 sgrid((ns + 255) / 256, M), ngrid((nf + 255) / 256, M);
+    double* out = E.h_iout;
+    auto put_int = [&](DevMem& dm, const std::vector<int>& v) {
+        HIPCHK(hipMemcpyAsync(dm.p, v.data(), M * sizeof(int), hipMemcpyHostToDevice, E.st));
+    };
+    auto reduce = [&](int n) {  // per-member sums of E.ipart (n entries each) into out[2 M]
+        hipLaunchKernelGGL(k_eimex_sum, dim3(M), dim3(256), 0, E.st, E.ipart.as<double>(), n, E.fail.as<int>(),
+                           E.iout.as<double>());
+        HIPCHK(hipMemcpyAsync(out, E.iout.p, (size_t)2 * M * 8, hipMemcpyDeviceToHost, E.st));
+        HIPCHK(hipStreamSynchronize(E.st));
+    };
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, E.st));
+    for (;;) {
+        // the members that take a step now, and its size (ending on the next save, t_off or tf)
+        bool any = false;
+        for (int m = 0; m < M; ++m) {
+            act[m] = 0;
+            if (done[m]) continue;
+            if (iters[m] >= maxiters) {
+                retcode[m] = 1;
+                done[m] = 1;
+                continue;
+            }
+            ++iters[m];
+            double s = std::min(tf, next_save[m]);
+            if (B.c.t_off > t[m] && B.c.t_off < s) s = B.c.t_off;
+            clipped[m] = dt[m] >= s - t[m];
+            h[m] = clipped[m] ? s - t[m] : dt[m];
+            hg[m] = 0.25 * h[m];
+            stop[m] = s;
+            okv[m] = 1;
+            act[m] = 1;
+            any = true;
+        }
+        if (!any) break;
+        HIPCHK(hipMemcpyAsync(E.ih.p, h.data(), M * 8, hipMemcpyHostToDevice, E.st));
+        HIPCHK(hipMemcpyAsync(E.ihg.p, hg.data(), M * 8, hipMemcpyHostToDevice, E.st));
+        std::vector<int> live(act);  // members in the step whose stages have all converged so far
+        for (int st = 0; st < 5; ++st) {
+            put_int(E.iact, live);
+            const double* Fg = st == 0 ? Fn : F + (size_t)(st - 1) * fst;
+            hipLaunchKernelGGL(k_eimex_base, sgrid, dim3(256), 0, E.st, ns, ld, st, E.ih.as<double>(),
+                               E.iact.as<int>(), un, F, (int64_t)fst, Fg, base, U);
+            for (int m = 0; m < M; ++m) tfix[m] = t[m] + cs[st] * h[m];
+            HIPCHK(hipMemcpyAsync(E.tfix.p, tfix.data(), M * 8, hipMemcpyHostToDevice, E.st));
+            nset = live;
+            std::fill(conv.begin(), conv.end(), 0);
+            for (int it = 0; it < kNewtonMax; ++it) {
+                put_int(E.iact, nset);
+                ens_gemm(E, U, ld);
+                hipLaunchKernelGGL(k_ens_stage, dim3(E.node_blocks(), M), dim3(256), 0, E.st, d, 0, U, (int64_t)ld,
+                                   Ft, (double*)nullptr, E.tfix.as<double>());
+                hipLaunchKernelGGL(k_eimex_newton, ngrid, dim3(256), 0, E.st, d, B.Kd.as<double>(), Ft, base,
+                                   E.ihg.as<double>(), E.irt.as<double>(), E.iact.as<int>(), un, U,
+                                   E.ipart.as<double>());
+                HIPCHK(hipGetLastError());
+                reduce(nf);
+                bool left = false;
+                for (int m = 0; m < M; ++m) {
+                    if (!nset[m]) continue;
+                    ++nrhs[m];
+                    if (out[2 * m + 1] != 0.0 || !std::isfinite(out[2 * m])) {
+                        nset[m] = 0;  // slip-rate solve failed
+                    } else if (std::sqrt(out[2 * m] / ns) < kNewtonTol) {
+                        nset[m] = 0;
+                        conv[m] = 1;
+                    } else {
+                        left = true;
+                    }
+                }
+                if (!left) break;
+            }
+            for (int m = 0; m < M; ++m)
+                if (live[m] && !conv[m]) {
+                    live[m] = 0;  // failed, or not converged after kNewtonMax iterations
+                    okv[m] = 0;
+                }
+            put_int(E.iact, live);
+            hipLaunchKernelGGL(k_eimex_stage_f, sgrid, dim3(256), 0, E.st, ns, ld, U, base, E.ihg.as<double>(),
+                               E.iact.as<int>(), F + (size_t)st * fst);
+        }
+        // error estimates of the members whose stages all converged (E.iact = live)
+        hipLaunchKernelGGL(k_eimex_err, sgrid, dim3(256), 0, E.st, B.c, ld, un, U, F, (int64_t)fst,
+                           E.ih.as<double>(), E.irt.as<double>(), E.iact.as<int>(), E.ipart.as<double>());
+        HIPCHK(hipGetLastError());
+        reduce(ns);
+        for (int m = 0; m < M; ++m) {
+            accv[m] = 0;
+            slot[m] = -1;
+            if (!act[m]) continue;
+            double err = 0.0;
+            bool ok = okv[m] != 0;
+            if (ok) {
+                err = std::sqrt(out[2 * m] / ns);
+                ok = std::isfinite(err);
+            }
+            if (ok && err <= 1.0) {
+                t[m] = clipped[m] ? stop[m] : t[m] + h[m];
+                ++naccept[m];
+                accv[m] = 1;
+                const bool on_save = std::fabs(t[m] - next_save[m]) <= sc_t;
+                const bool at_end = t[m] >= tf;
+                if ((on_save || at_end) && nsaved[m] < nsave_max) {
+                    slot[m] = nsaved[m];
+                    if (!on_save) end_save[m] = 1;
+                    ++nsaved[m];
+                }
+                if (on_save) next_save[m] = t0 + (double)nsaved[m] * saveat;
+                const double fac = std::min(4.0, std::max(0.2, 0.9 * std::pow(std::max(err, 1e-10), -0.25)));
+                dt[m] = clipped[m] ? std::max(dt[m], h[m] * fac) : h[m] * fac;
+                if (at_end) done[m] = 1;
+            } else {
+                ++nreject[m];
+                dt[m] = ok ? h[m] * std::max(0.2, 0.9 * std::pow(err, -0.25)) : h[m] * 0.25;
+                if (dt[m] < 1e-12 * std::max(1.0, std::fabs(t[m]))) {
+                    retcode[m] = 2;
+                    done[m] = 1;
+                }
+            }
+        }
+        put_int(E.iacc, accv);
+        HIPCHK(hipMemcpyAsync(E.islot.p, slot.data(), M * sizeof(long long), hipMemcpyHostToDevice, E.st));
+        hipLaunchKernelGGL(k_eimex_commit, sgrid, dim3(256), 0, E.st, ns, ld, E.iacc.as<int>(),
+                           E.islot.as<long long>(), (int64_t)E.save_cap, U, F + (size_t)4 * fst, un, Fn,
+                           E.saves.as<double>());
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(e1, E.st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (loop_ms) *loop_ms = ms;
+    // the saves' bookkeeping where eqd_bp8_ensemble_saved reads it
+    int failed = -1;
+    int64_t nsv = nsaved[0];
+    for (int m = 0; m < M; ++m) {
+        Ctl& c = E.h_ctl[m];
+        std::memset(&c, 0, sizeof(Ctl));
+        c.t0 = t0;
+        c.tf = tf;
+        c.saveat = saveat;
+        c.nsaved = nsaved[m];
+        c.end_save = end_save[m];
+        c.naccept = naccept[m];
+        c.nreject = nreject[m];
+        c.nrhs = nrhs[m];
+        c.retcode = retcode[m];
+        c.done = 1;
+        nsv = std::min<int64_t>(nsv, nsaved[m]);
+        if (retcode[m] == 2 && failed < 0) failed = m;
+        if (stats) {
+            stats[m].naccept = naccept[m];
+            stats[m].nreject = nreject[m];
+            stats[m].nrhs = nrhs[m];
+            stats[m].nsaved = nsaved[m];
+            stats[m].retcode = retcode[m];
+            stats[m].ms = ms;
+        }
+    }
+    E.nsaved = nsv;
+    if (failed >= 0)
+        throw Error(EQD_ERR_NEWTON,
+                    "implicit stage solve failed at the minimum step (member " + std::to_string(failed) + ")");
 }
 
 }  // namespace
@@ -2461,6 +2908,15 @@ eqd_status eqd_bp8_ensemble_integrate(eqd_bp8_ensemble* e, double t0, double tf,
     return guard([&] {
         if (!e) throw Error(EQD_ERR_ARG, "null ensemble");
         ens_integrate(*e, t0, tf, saveat, reltol, member_reltol, maxiters, nsave_max, stats, loop_ms);
+    });
+}
+
+eqd_status eqd_bp8_ensemble_integrate_imex(eqd_bp8_ensemble* e, double t0, double tf, double saveat, double reltol,
+                                           const double* member_reltol, int64_t maxiters, int64_t nsave_max,
+                                           eqd_bp8_stats* stats, double* loop_ms) {
+    return guard([&] {
+        if (!e) throw Error(EQD_ERR_ARG, "null ensemble");
+        ens_integrate_imex(*e, t0, tf, saveat, reltol, member_reltol, maxiters, nsave_max, stats, loop_ms);
     });
 }
 

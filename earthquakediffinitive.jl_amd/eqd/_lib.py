@@ -38,7 +38,7 @@ EXPORTED = [
     "eqd_server_submit_forward", "eqd_server_step", "eqd_server_completed", "eqd_server_result",
     "eqd_server_grid_info",
     "eqd_bp8_ensemble_create", "eqd_bp8_ensemble_destroy", "eqd_bp8_ensemble_integrate",
-    "eqd_bp8_ensemble_saved", "eqd_bp8_ensemble_saved_vmax", "eqd_bp8_ensemble_gemm_timing",
+    "eqd_bp8_ensemble_integrate_imex", "eqd_bp8_ensemble_saved", "eqd_bp8_ensemble_saved_vmax", "eqd_bp8_ensemble_gemm_timing",
 ]
 
 INJECTION_GAUSSIAN, INJECTION_PEACEMAN = 0, 1
@@ -152,6 +152,7 @@ def lib():
         L.eqd_bp8_ensemble_destroy.argtypes = [vp]
         L.eqd_bp8_ensemble_integrate.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, vp,
                                                  C.c_int64, C.c_int64, C.POINTER(BP8Stats), pd]
+        L.eqd_bp8_ensemble_integrate_imex.argtypes = L.eqd_bp8_ensemble_integrate.argtypes
         L.eqd_bp8_ensemble_saved.argtypes = [vp, C.c_int, vp, vp, C.POINTER(C.c_int64)]
         L.eqd_bp8_ensemble_saved_vmax.argtypes = [vp, vp, C.c_int64, pd]
         L.eqd_bp8_ensemble_gemm_timing.argtypes = [vp, C.c_int, pd, pd]

@@ -62,8 +62,12 @@ class Ensemble:
             self.handle = None
 
     def run(self, *, tspan=None, reltol: float = 1e-8, member_reltol=None, saveat: float = 3600.0,
-            maxiters: int = 100000):
-        """Integrates every member; returns the per-member BP8Stats list."""
+            maxiters: int = 100000, alg: str = "tsit5"):
+        """Integrates every member; returns the per-member BP8Stats list. alg = "tsit5" (run_bp8's
+        integrator, src/BP8.jl:442-470) or "imex" (the implicit-local integrator of run_bp8's
+        alg="imex", for the stiff Peaceman-well variant)."""
+        if alg not in ("tsit5", "imex"):
+            raise EqdError(_lib.EQD_ERR_ARG, f"alg must be tsit5 or imex, got {alg}")
         par = self.model.par
         t0, tf = (0.0, par["t_f"]) if tspan is None else (float(tspan[0]), float(tspan[1]))
         nmax = int(math.floor((tf - t0) / saveat + 1e-9)) + 2
@@ -74,9 +78,9 @@ class Ensemble:
             rt = np.ascontiguousarray(member_reltol, dtype=np.float64)
             if rt.shape != (self.members,):
                 raise EqdError(_lib.EQD_ERR_ARG, "member_reltol needs one value per member")
-        check(lib().eqd_bp8_ensemble_integrate(self.handle, t0, tf, float(saveat), float(reltol),
-                                               None if rt is None else rt.ctypes.data, int(maxiters), nmax,
-                                               st, C.byref(ms)))
+        fn = lib().eqd_bp8_ensemble_integrate if alg == "tsit5" else lib().eqd_bp8_ensemble_integrate_imex
+        check(fn(self.handle, t0, tf, float(saveat), float(reltol), None if rt is None else rt.ctypes.data,
+                 int(maxiters), nmax, st, C.byref(ms)))
         self.stats = list(st)
         self.loop_ms = ms.value
         return self.stats

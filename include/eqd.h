@@ -309,6 +309,15 @@ eqd_status eqd_bp8_ensemble_destroy(eqd_bp8_ensemble* e);
 eqd_status eqd_bp8_ensemble_integrate(eqd_bp8_ensemble* e, double t0, double tf, double saveat, double reltol,
                                       const double* member_reltol, int64_t maxiters, int64_t nsave_max,
                                       eqd_bp8_stats* stats, double* loop_ms);
+/* Every member over (t0, tf) with the implicit-local integrator, as eqd_bp8_integrate_imex does
+ * for one trajectory (the stiff Peaceman-well variant, src/BP8.jl:403-433 with the well of
+ * src/PorePressure.jl:125-158): each member keeps its own step size, Newton iterations, error
+ * control and saves; each Newton iteration forms K U and rhs! over all members at once. Same
+ * arguments, stats and errors as eqd_bp8_ensemble_integrate (retcode 2: a member's stage solve
+ * failed at the minimum step). No counterpart in the reference, which integrates with Tsit5. */
+eqd_status eqd_bp8_ensemble_integrate_imex(eqd_bp8_ensemble* e, double t0, double tf, double saveat,
+                                           double reltol, const double* member_reltol, int64_t maxiters,
+                                           int64_t nsave_max, eqd_bp8_stats* stats, double* loop_ms);
 /* A member's save times ts[nsaved] and states us[nsaved * state_len] (host; either may be NULL). */
 eqd_status eqd_bp8_ensemble_saved(const eqd_bp8_ensemble* e, int member, double* ts, double* us,
                                   int64_t* nsaved);
