@@ -421,18 +421,26 @@ function DeviceEnsemble(m::DeviceBP8Model, members::Integer; Q0=nothing, a=nothi
     return e
 end
 
-"Integrates every member (run_bp8's controller per member); returns the members' stats."
+"""Integrates every member (run_bp8's controller per member); returns the members' stats.
+alg=:imex runs the implicit-local integrator over the members (eqd_bp8_ensemble_integrate_imex)."""
 function run_ensemble!(e::DeviceEnsemble; tspan=(0.0, e.model.par.t_f), reltol=1e-8, member_reltol=nothing,
-                       saveat=3600.0, maxiters=100000)
+                       saveat=3600.0, maxiters=100000, alg::Symbol=:tsit5)
+    alg in (:tsit5, :imex) || error("alg must be :tsit5 or :imex, got $alg")
     t0, tf = Float64.(tspan)
     nmax = floor(Int, (tf - t0) / saveat + 1e-9) + 2
     st = Vector{BP8StatsC}(undef, e.members)
     rt = member_reltol === nothing ? C_NULL : Float64.(collect(member_reltol))
     ms = Ref{Cdouble}(0)
     GC.@preserve rt begin
-        check(ccall((:eqd_bp8_ensemble_integrate, libeqd), Cint,
-                    (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Ptr{Cdouble}, Int64, Int64, Ptr{BP8StatsC},
-                     Ref{Cdouble}), e.handle, t0, tf, saveat, reltol, rt, maxiters, nmax, st, ms))
+        if alg === :tsit5
+            check(ccall((:eqd_bp8_ensemble_integrate, libeqd), Cint,
+                        (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Ptr{Cdouble}, Int64, Int64, Ptr{BP8StatsC},
+                         Ref{Cdouble}), e.handle, t0, tf, saveat, reltol, rt, maxiters, nmax, st, ms))
+        else
+            check(ccall((:eqd_bp8_ensemble_integrate_imex, libeqd), Cint,
+                        (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Ptr{Cdouble}, Int64, Int64, Ptr{BP8StatsC},
+                         Ref{Cdouble}), e.handle, t0, tf, saveat, reltol, rt, maxiters, nmax, st, ms))
+        end
     end
     e.stats = st
     return st
