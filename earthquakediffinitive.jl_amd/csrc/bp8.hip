@@ -862,6 +862,12 @@ struct EnsDev {
     const double* D;       // splits x 2nf x M: partial K y of the current stage
     int splits;
     double* tv;            // null, or 4 x nf x M: T2, T3, V2, V3 of the last evaluation (implicit-local)
+    const int* rmask;      // null, or M: the members an evaluation covers (implicit-local)
+};
+
+// A member mask in the form the FFT K S kernels skip members by (done = leave the member alone).
+struct SkipFlag {
+    int done;
 };
 
 // One rhs! per (node, member): thread i over nodes, member m = blockIdx.y. d.D holds the
@@ -873,7 +879,7 @@ __global__ __launch_bounds__(256) void k_ens_stage(EnsDev d, int stage, const do
     const int m = blockIdx.y;
     const int i = blockIdx.x * 256 + threadIdx.x;
     const Ctl& cm = d.ctl[m];
-    const bool run = stage == 0 || !cm.done;
+    const bool run = (stage == 0 || !cm.done) && (!d.rmask || d.rmask[m]);
     double ssum = 0.0;
     if (run && i < c.nf) {
         const double dt = stage != 0 ? cm.dt : 0.0;
@@ -1526,7 +1532,7 @@ struct eqd_bp8_ensemble {
     DevMem Kc, pa, psig, pQ0, psrc, Vprev, Vm, fail, k, uprev, unew, ya, yb, D, partial, ctl, saves, tfix;
     // the implicit-local integrator: tractions and slip rates of the last evaluation, per-entry
     // partials, per-member step sizes, tolerances, masks, save slots and reduction results
-    DevMem itv, ipart, ih, ihg, irt, iact, iacc, islot, iout;
+    DevMem itv, ipart, ih, ihg, irt, iact, iacc, islot, iout, iskip;
     double* h_iout = nullptr;
     int64_t save_cap = 0, nsaved = 0;
     hipStream_t st = nullptr;
@@ -1569,6 +1575,7 @@ struct eqd_bp8_ensemble {
         d.D = D.as<double>();
         d.splits = splits;
         d.tv = nullptr;
+        d.rmask = nullptr;
         return d;
     }
 };
@@ -2307,6 +2314,7 @@ void ens_integrate_imex(eqd_bp8_ensemble& E, double t0, double tf, double saveat
         E.iacc.alloc((size_t)M * sizeof(int));
         E.islot.alloc((size_t)M * sizeof(long long));
         E.iout.alloc((size_t)2 * M * 8);
+        E.iskip.alloc((size_t)M * sizeof(int));
         HIPCHK(hipHostMalloc((void**)&E.h_iout, (size_t)2 * M * 8, hipHostMallocDefault));
     }
     if (!B.Kd.p) {
@@ -2320,6 +2328,7 @@ void ens_integrate_imex(eqd_bp8_ensemble& E, double t0, double tf, double saveat
     HIPCHK(hipMemcpyAsync(E.irt.p, rtol.data(), M * 8, hipMemcpyHostToDevice, E.st));
     EnsDev d = E.dev();
     d.tv = E.itv.as<double>();
+    d.rmask = E.iact.as<int>();  // an evaluation covers the members still iterating
     const size_t fst = (size_t)ld * M;
     double* F = E.k.as<double>();  // stage derivatives F_0 .. F_4 in k[0..4]
     double* Fn = F + 5 * fst;      // f(u_n)
@@ -2333,7 +2342,7 @@ void ens_integrate_imex(eqd_bp8_ensemble& E, double t0, double tf, double saveat
     std::vector<double> t(M, t0), dt(dts), next_save(M, t0 + saveat), h(M, 0.0), hg(M, 0.0), stop(M, tf), tfix(M);
     std::vector<int64_t> nsaved(M, 1), naccept(M, 0), nreject(M, 0), nrhs(M, 2), iters(M, 0);
     std::vector<int> retcode(M, 0), done(M, 0), act(M, 0), okv(M, 0), clipped(M, 0), nset(M, 0), conv(M, 0),
-        accv(M, 0), end_save(M, 0);
+        accv(M, 0), end_save(M, 0), skip(M, 0);
     std::vector<long long> slot(M, -1);
     const double sc_t = 1e-9 * saveat;
     static const double cs[5] = {0.25, 0.75, 11.0 / 20.0, 0.5, 1.0};
@@ -2389,7 +2398,14 @@ void ens_integrate_imex(eqd_bp8_ensemble& E, double t0, double tf, double saveat
             std::fill(conv.begin(), conv.end(), 0);
             for (int it = 0; it < kNewtonMax; ++it) {
                 put_int(E.iact, nset);
-                ens_gemm(E, U, ld);
+                if (E.use_fft) {  // K U of the members still iterating
+                    for (int m = 0; m < M; ++m) skip[m] = !nset[m];
+                    put_int(E.iskip, skip);
+                    fft_ks_launch(E.fft, E.st, U, (int64_t)ld, nf, E.D.as<double>(), (int64_t)B.c.n2, 1, (int64_t)nf,
+                                  M, reinterpret_cast<const SkipFlag*>(E.iskip.as<int>()), 1);
+                } else {
+                    ens_gemm(E, U, ld);
+                }
                 hipLaunchKernelGGL(k_ens_stage, dim3(E.node_blocks(), M), dim3(256), 0, E.st, d, 0, U, (int64_t)ld,
                                    Ft, (double*)nullptr, E.tfix.as<double>());
                 hipLaunchKernelGGL(k_eimex_newton, ngrid, dim3(256), 0, E.st, d, B.Kd.as<double>(), Ft, base,
