@@ -1311,10 +1311,7 @@ __global__ void k_eimex_newton(EnsDev d, const double* __restrict__ Kd, const do
     const int nf = c.nf;
     if (i >= nf) return;
     const size_t nm = (size_t)m * nf + i;
-    if (!act[m]) {
-        partial[nm] = 0.0;
-        return;
-    }
+    if (!act[m]) return;
     const double hg = hgv[m], reltol = rtol[m], a = d.pa[m];
     const size_t um = (size_t)m * d.ldU;
     const size_t e0 = um + i, e1 = um + nf + i, e2 = um + 2 * nf + i, e3 = um + 3 * nf + i;
@@ -1415,6 +1412,57 @@ __global__ __launch_bounds__(256) void k_eimex_sum(const double* __restrict__ v,
         out[2 * m] = red[0];
         out[2 * m + 1] = (double)fail[m];
         fail[m] = 0;
+    }
+}
+
+// The Newton test of one iteration for the members still iterating (nset), on the device: the
+// member's sum of squared scaled updates (v, n entries per member, summed as k_eimex_sum does)
+// against tol, or a failed slip-rate solve. A member that stops leaves nset (conv = 1 when it
+// converged), skip follows nset for the FFT K S, cnt counts the member's evaluations.
+__global__ __launch_bounds__(256) void k_eimex_decide(const double* __restrict__ v, int n, int ns, double tol,
+                                                       int* __restrict__ fail, int* __restrict__ nset,
+                                                       int* __restrict__ skip, int* __restrict__ conv,
+                                                       long long* __restrict__ cnt) {
+    __shared__ double red[256];
+    const int m = blockIdx.x;
+    if (!nset[m]) return;
+    const double* vm = v + (size_t)m * n;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) acc += vm[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ++cnt[m];
+        const double s = red[0];
+        const int fl = fail[m];
+        fail[m] = 0;
+        if (fl != 0 || !isfinite(s)) {
+            nset[m] = 0;
+        } else if (sqrt(s / ns) < tol) {
+            nset[m] = 0;
+            conv[m] = 1;
+        }
+        skip[m] = !nset[m];
+    }
+}
+
+// Member masks around a stage's Newton iterations. start: every member of the step still live
+// iterates (nset = live, skip = !live, conv = 0); end: a member whose stage did not converge is
+// no longer live.
+__global__ void k_eimex_masks(int M, int start, int* __restrict__ live, int* __restrict__ nset,
+                              int* __restrict__ skip, int* __restrict__ conv) {
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    if (m >= M) return;
+    if (start) {
+        nset[m] = live[m];
+        skip[m] = !live[m];
+        conv[m] = 0;
+    } else {
+        live[m] = live[m] && conv[m];
     }
 }
 
@@ -1532,8 +1580,9 @@ struct eqd_bp8_ensemble {
     DevMem Kc, pa, psig, pQ0, psrc, Vprev, Vm, fail, k, uprev, unew, ya, yb, D, partial, ctl, saves, tfix;
     // the implicit-local integrator: tractions and slip rates of the last evaluation, per-entry
     // partials, per-member step sizes, tolerances, masks, save slots and reduction results
-    DevMem itv, ipart, ih, ihg, irt, iact, iacc, islot, iout, iskip;
+    DevMem itv, ipart, ih, ihg, irt, iact, iacc, islot, iout, iskip, ilive, iconv, icnt;
     double* h_iout = nullptr;
+    int* h_imask = nullptr;  // 2 M: the iterating set of the last iteration, the live set of a step
     int64_t save_cap = 0, nsaved = 0;
     hipStream_t st = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1547,6 +1596,7 @@ struct eqd_bp8_ensemble {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (h_ctl) (void)hipHostFree(h_ctl);
         if (h_iout) (void)hipHostFree(h_iout);
+        if (h_imask) (void)hipHostFree(h_imask);
         if (st) (void)hipStreamDestroy(st);
     }
     int node_blocks() const { return (base->c.nf + 255) / 256; }
@@ -2315,7 +2365,11 @@ void ens_integrate_imex(eqd_bp8_ensemble& E, double t0, double tf, double saveat
         E.islot.alloc((size_t)M * sizeof(long long));
         E.iout.alloc((size_t)2 * M * 8);
         E.iskip.alloc((size_t)M * sizeof(int));
+        E.ilive.alloc((size_t)M * sizeof(int));
+        E.iconv.alloc((size_t)M * sizeof(int));
+        E.icnt.alloc((size_t)M * sizeof(long long));
         HIPCHK(hipHostMalloc((void**)&E.h_iout, (size_t)2 * M * 8, hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&E.h_imask, (size_t)2 * M * sizeof(int), hipHostMallocDefault));
     }
     if (!B.Kd.p) {
         B.Kd.alloc((size_t)4 * nf * 8);
@@ -2341,13 +2395,13 @@ void ens_integrate_imex(eqd_bp8_ensemble& E, double t0, double tf, double saveat
                             hipMemcpyDeviceToDevice, E.st));
     std::vector<double> t(M, t0), dt(dts), next_save(M, t0 + saveat), h(M, 0.0), hg(M, 0.0), stop(M, tf), tfix(M);
     std::vector<int64_t> nsaved(M, 1), naccept(M, 0), nreject(M, 0), nrhs(M, 2), iters(M, 0);
-    std::vector<int> retcode(M, 0), done(M, 0), act(M, 0), okv(M, 0), clipped(M, 0), nset(M, 0), conv(M, 0),
-        accv(M, 0), end_save(M, 0), skip(M, 0);
+    std::vector<int> retcode(M, 0), done(M, 0), act(M, 0), okv(M, 0), clipped(M, 0), accv(M, 0), end_save(M, 0);
     std::vector<long long> slot(M, -1);
     const double sc_t = 1e-9 * saveat;
     static const double cs[5] = {0.25, 0.75, 11.0 / 20.0, 0.5, 1.0};
-    const dim3 sgrid((ns + 255) / 256, M), ngrid((nf + 255) / 256, M);
+    const dim3 sgrid((ns + 255) / 256, M), ngrid((nf + 255) / 256, M), mgrid((M + 255) / 256);
     double* out = E.h_iout;
+    int stage_its[5] = {1, 1, 1, 1, 1};  // Newton iterations each stage took in the last step
     auto put_int = [&](DevMem& dm, const std::vector<int>& v) {
         HIPCHK(hipMemcpyAsync(dm.p, v.data(), M * sizeof(int), hipMemcpyHostToDevice, E.st));
     };
@@ -2357,6 +2411,7 @@ void ens_integrate_imex(eqd_bp8_ensemble& E, double t0, double tf, double saveat
         HIPCHK(hipMemcpyAsync(out, E.iout.p, (size_t)2 * M * 8, hipMemcpyDeviceToHost, E.st));
         HIPCHK(hipStreamSynchronize(E.st));
     };
+    HIPCHK(hipMemsetAsync(E.icnt.p, 0, M * sizeof(long long), E.st));
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -2379,69 +2434,69 @@ void ens_integrate_imex(eqd_bp8_ensemble& E, double t0, double tf, double saveat
             h[m] = clipped[m] ? s - t[m] : dt[m];
             hg[m] = 0.25 * h[m];
             stop[m] = s;
-            okv[m] = 1;
             act[m] = 1;
             any = true;
         }
         if (!any) break;
         HIPCHK(hipMemcpyAsync(E.ih.p, h.data(), M * 8, hipMemcpyHostToDevice, E.st));
         HIPCHK(hipMemcpyAsync(E.ihg.p, hg.data(), M * 8, hipMemcpyHostToDevice, E.st));
-        std::vector<int> live(act);  // members in the step whose stages have all converged so far
+        // E.ilive: the members in the step whose stages have all converged so far; E.iact: those
+        // still iterating in the current stage. The Newton test runs on the device
+        // (k_eimex_decide), and the host reads the iterating set back after a run of iterations,
+        // as many as the stage took in the last step minus one, then one at a time: iterations
+        // past a member's exit leave it untouched, so the results are those of testing after
+        // every iteration.
+        put_int(E.ilive, act);
         for (int st = 0; st < 5; ++st) {
-            put_int(E.iact, live);
             const double* Fg = st == 0 ? Fn : F + (size_t)(st - 1) * fst;
             hipLaunchKernelGGL(k_eimex_base, sgrid, dim3(256), 0, E.st, ns, ld, st, E.ih.as<double>(),
-                               E.iact.as<int>(), un, F, (int64_t)fst, Fg, base, U);
+                               E.ilive.as<int>(), un, F, (int64_t)fst, Fg, base, U);
             for (int m = 0; m < M; ++m) tfix[m] = t[m] + cs[st] * h[m];
             HIPCHK(hipMemcpyAsync(E.tfix.p, tfix.data(), M * 8, hipMemcpyHostToDevice, E.st));
-            nset = live;
-            std::fill(conv.begin(), conv.end(), 0);
-            for (int it = 0; it < kNewtonMax; ++it) {
-                put_int(E.iact, nset);
-                if (E.use_fft) {  // K U of the members still iterating
-                    for (int m = 0; m < M; ++m) skip[m] = !nset[m];
-                    put_int(E.iskip, skip);
-                    fft_ks_launch(E.fft, E.st, U, (int64_t)ld, nf, E.D.as<double>(), (int64_t)B.c.n2, 1, (int64_t)nf,
-                                  M, reinterpret_cast<const SkipFlag*>(E.iskip.as<int>()), 1);
-                } else {
-                    ens_gemm(E, U, ld);
-                }
-                hipLaunchKernelGGL(k_ens_stage, dim3(E.node_blocks(), M), dim3(256), 0, E.st, d, 0, U, (int64_t)ld,
-                                   Ft, (double*)nullptr, E.tfix.as<double>());
-                hipLaunchKernelGGL(k_eimex_newton, ngrid, dim3(256), 0, E.st, d, B.Kd.as<double>(), Ft, base,
-                                   E.ihg.as<double>(), E.irt.as<double>(), E.iact.as<int>(), un, U,
-                                   E.ipart.as<double>());
-                HIPCHK(hipGetLastError());
-                reduce(nf);
-                bool left = false;
-                for (int m = 0; m < M; ++m) {
-                    if (!nset[m]) continue;
-                    ++nrhs[m];
-                    if (out[2 * m + 1] != 0.0 || !std::isfinite(out[2 * m])) {
-                        nset[m] = 0;  // slip-rate solve failed
-                    } else if (std::sqrt(out[2 * m] / ns) < kNewtonTol) {
-                        nset[m] = 0;
-                        conv[m] = 1;
+            hipLaunchKernelGGL(k_eimex_masks, mgrid, dim3(256), 0, E.st, M, 1, E.ilive.as<int>(), E.iact.as<int>(),
+                               E.iskip.as<int>(), E.iconv.as<int>());
+            int done_it = 0, run = std::max(1, stage_its[st] - 1);
+            while (done_it < kNewtonMax) {
+                const int nrun = std::min(run, kNewtonMax - done_it);
+                for (int it = 0; it < nrun; ++it) {
+                    if (E.use_fft) {  // K U of the members still iterating
+                        fft_ks_launch(E.fft, E.st, U, (int64_t)ld, nf, E.D.as<double>(), (int64_t)B.c.n2, 1,
+                                      (int64_t)nf, M, reinterpret_cast<const SkipFlag*>(E.iskip.as<int>()), 1);
                     } else {
-                        left = true;
+                        ens_gemm(E, U, ld);
                     }
+                    hipLaunchKernelGGL(k_ens_stage, dim3(E.node_blocks(), M), dim3(256), 0, E.st, d, 0, U,
+                                       (int64_t)ld, Ft, (double*)nullptr, E.tfix.as<double>());
+                    hipLaunchKernelGGL(k_eimex_newton, ngrid, dim3(256), 0, E.st, d, B.Kd.as<double>(), Ft, base,
+                                       E.ihg.as<double>(), E.irt.as<double>(), E.iact.as<int>(), un, U,
+                                       E.ipart.as<double>());
+                    hipLaunchKernelGGL(k_eimex_decide, dim3(M), dim3(256), 0, E.st, E.ipart.as<double>(), nf, ns,
+                                       kNewtonTol, E.fail.as<int>(), E.iact.as<int>(), E.iskip.as<int>(),
+                                       E.iconv.as<int>(), E.icnt.as<long long>());
+                    HIPCHK(hipGetLastError());
                 }
+                done_it += nrun;
+                run = 1;
+                HIPCHK(hipMemcpyAsync(E.h_imask, E.iact.p, M * sizeof(int), hipMemcpyDeviceToHost, E.st));
+                HIPCHK(hipStreamSynchronize(E.st));
+                bool left = false;
+                for (int m = 0; m < M && !left; ++m) left = E.h_imask[m] != 0;
                 if (!left) break;
             }
-            for (int m = 0; m < M; ++m)
-                if (live[m] && !conv[m]) {
-                    live[m] = 0;  // failed, or not converged after kNewtonMax iterations
-                    okv[m] = 0;
-                }
-            put_int(E.iact, live);
+            stage_its[st] = done_it;
+            // a member whose stage failed or did not converge after kNewtonMax iterations drops out
+            hipLaunchKernelGGL(k_eimex_masks, mgrid, dim3(256), 0, E.st, M, 0, E.ilive.as<int>(), E.iact.as<int>(),
+                               E.iskip.as<int>(), E.iconv.as<int>());
             hipLaunchKernelGGL(k_eimex_stage_f, sgrid, dim3(256), 0, E.st, ns, ld, U, base, E.ihg.as<double>(),
-                               E.iact.as<int>(), F + (size_t)st * fst);
+                               E.ilive.as<int>(), F + (size_t)st * fst);
         }
-        // error estimates of the members whose stages all converged (E.iact = live)
+        // error estimates of the members whose stages all converged (E.ilive)
         hipLaunchKernelGGL(k_eimex_err, sgrid, dim3(256), 0, E.st, B.c, ld, un, U, F, (int64_t)fst,
-                           E.ih.as<double>(), E.irt.as<double>(), E.iact.as<int>(), E.ipart.as<double>());
+                           E.ih.as<double>(), E.irt.as<double>(), E.ilive.as<int>(), E.ipart.as<double>());
         HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(E.h_imask + M, E.ilive.p, M * sizeof(int), hipMemcpyDeviceToHost, E.st));
         reduce(ns);
+        for (int m = 0; m < M; ++m) okv[m] = act[m] && E.h_imask[M + m];
         for (int m = 0; m < M; ++m) {
             accv[m] = 0;
             slot[m] = -1;
@@ -2490,6 +2545,11 @@ void ens_integrate_imex(eqd_bp8_ensemble& E, double t0, double tf, double saveat
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (loop_ms) *loop_ms = ms;
+    {
+        std::vector<long long> cnt(M);  // evaluations inside the Newton iterations
+        HIPCHK(hipMemcpy(cnt.data(), E.icnt.p, M * sizeof(long long), hipMemcpyDeviceToHost));
+        for (int m = 0; m < M; ++m) nrhs[m] += cnt[m];
+    }
     // the saves' bookkeeping where eqd_bp8_ensemble_saved reads it
     int failed = -1;
     int64_t nsv = nsaved[0];
