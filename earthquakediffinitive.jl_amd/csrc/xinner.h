@@ -785,8 +785,10 @@ __global__ __launch_bounds__(kBlock) void k_xi_update(int NX, int NY, int mface,
         const size_t off = xi_tile_elems(NX, NY) * tile + (size_t)sl.cq * 4 * sl.B;
         const int nbl = NY * NY * 3;
         const int G = (int)gridDim.x;
-        // EQD_XI_UPD_U (line, component) blocks per step, all loaded before any is stored
-        for (int bl0 = g; bl0 < nbl; bl0 += EQD_XI_UPD_U * G) {
+        // EQD_XI_UPD_U (line, component) blocks per step, all loaded before any is stored; a
+        // thread whose four columns are all done leaves r alone
+        const bool anya = act[0] || act[1] || act[2] || act[3];
+        for (int bl0 = g; anya && bl0 < nbl; bl0 += EQD_XI_UPD_U * G) {
             for (int s = 0; s < sl.S; ++s) {
                 const int m = sl.n + 16 * s;
                 if (m >= sl.B) break;
