@@ -895,10 +895,12 @@ int effective_k(const eqd_solver* s, int) {
 
 // Solves a batch of columns whose slip fields are produced by `fill` into
 // ws->fields (host or device), then extracts tractions into Kdev[:, outpos].
+// Columns [0, split) and [split, n) run as separate batch sequences (split < 0: one sequence).
 template <typename Fill>
 void run_columns(eqd_solver* s, const std::vector<int64_t>& colid, const std::vector<int64_t>& outpos, Fill&& fill,
-                 double* Kdev, int64_t ldK) {
+                 double* Kdev, int64_t ldK, int split = -1) {
     const int n = (int)colid.size();
+    if (split < 0 || split > n) split = n;
     const int prec = s->o.engine == EQD_ENGINE_PLAIN ? EQD_PREC_FP64 : s->o.precision;
     int per = s->o.max_cols_in_flight > 0 ? s->o.max_cols_in_flight : n;
     {  // bound by device memory
@@ -943,7 +945,7 @@ void run_columns(eqd_solver* s, const std::vector<int64_t>& colid, const std::ve
     }
     s->last_k = K;
     // tile_group balances any tile count over the 8 XCDs, so no padding tiles
-    const int ntiles = (std::min(per, n) + K - 1) / K;
+    const int ntiles = (std::min(per, std::max(split, n - split)) + K - 1) / K;
     ensure_ws(s, K, ntiles, prec);
     Workspace& w = *s->ws;
     const int slots = ntiles * K;
@@ -951,9 +953,13 @@ void run_columns(eqd_solver* s, const std::vector<int64_t>& colid, const std::ve
     s->kt->reset();
     double ms = 0;
     int64_t launches = 0, tile_iters = 0;
-    for (int base = 0; base < n; base += slots) {
+    std::vector<std::pair<int, int>> batches;  // [base, end) of each batch
+    for (int b = 0; b < split; b += slots) batches.emplace_back(b, std::min(b + slots, split));
+    for (int b = split; b < n; b += slots) batches.emplace_back(b, std::min(b + slots, n));
+    for (const auto& bt : batches) {
+        const int base = bt.first, end = bt.second;
         std::vector<int64_t> cid(slots, -1), opos(slots, -1);
-        for (int i = 0; i < slots && base + i < n; ++i) {
+        for (int i = 0; i < slots && base + i < end; ++i) {
             cid[i] = colid[base + i];
             opos[i] = outpos[base + i];
         }
@@ -972,7 +978,7 @@ void run_columns(eqd_solver* s, const std::vector<int64_t>& colid, const std::ve
         tile_iters += s->last_tile_iters;
         std::vector<ColState> hs(slots);
         HIPCHK(hipMemcpy(hs.data(), w.cs.p, slots * sizeof(ColState), hipMemcpyDeviceToHost));
-        for (int i = 0; i < slots && base + i < n; ++i) {
+        for (int i = 0; i < slots && base + i < end; ++i) {
             const ColState& c = hs[i];
             s->st.solves += 1;
             s->st.iterations += c.iters;
@@ -1042,7 +1048,35 @@ void stiffness(eqd_solver* s, int mode, const int* cols, int ncols, double* Kdev
             }
         }
         if (cid.empty()) return;
-        run_columns(s, cid, opos, [&](Workspace& w, int, int slots) { unit_fill(s, w, slots); }, Kdev, ldK);
+        int split = -1;
+        // A unit slip s2 on face row iz, or s3 on face column iy, outside the interior range
+        // [1, n23 - 2] gives b = HP(D+SAT) chi = 0 (k_rhs reads s2 only on interior rows and s3
+        // only on interior columns): such columns converge at iteration 0. They run as a batch
+        // of their own after the others, so they hold no slots in tiles that iterate and the
+        // iterating batch has fewer tiles (l_f = L_fault at dz = 50 m: 132 of the 2178 columns,
+        // 32 tiles instead of 35). Each column's result does not depend on its tile or batch,
+        // so K is unchanged.
+        {
+            const int n23 = op.g.n23;
+            auto zero_rhs = [&](int64_t c) {
+                const int k = (int)(c < nf ? c : c - nf);
+                const int f = op.i2lo + (c < nf ? k / op.n2f : k % op.n2f);
+                return f < 1 || f > n23 - 2;
+            };
+            std::vector<int64_t> c2, o2;
+            size_t nz = 0;
+            for (size_t i = 0; i < cid.size(); ++i) nz += !zero_rhs(cid[i]);
+            for (int pass = 0; pass < 2; ++pass)
+                for (size_t i = 0; i < cid.size(); ++i)
+                    if (zero_rhs(cid[i]) == (pass == 1)) {
+                        c2.push_back(cid[i]);
+                        o2.push_back(opos[i]);
+                    }
+            cid.swap(c2);
+            opos.swap(o2);
+            split = (int)nz;
+        }
+        run_columns(s, cid, opos, [&](Workspace& w, int, int slots) { unit_fill(s, w, slots); }, Kdev, ldK, split);
         return;
     }
     if (mode != EQD_STIFFNESS_TOEPLITZ) throw Error(EQD_ERR_ARG, "mode must be EXACT or TOEPLITZ");
