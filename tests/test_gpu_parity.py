@@ -148,6 +148,31 @@ def test_batching_is_bitwise_invariant(precision, tile_k, name):
     assert np.array_equal(eqd.fault_stiffness(fe3, cols=cols), K_all[:, cols])
 
 
+def test_zero_rhs_columns_run_apart():
+    """With l_f = L_fault, a unit slip s2 on a boundary face row or s3 on a boundary face column
+    gives b = HP(D+SAT)chi = 0; those columns run as a batch of their own after the iterating
+    ones. A build of only them takes no CG iteration, the rest take the full build's iterations,
+    and every column is the same bits whichever batch it ran in (and matches the oracle)."""
+    n23 = CASES["ring"][1]
+    fe, o = pair("ring", precision="mixed", tile_k=64)
+    nf = o.nf
+    assert nf == n23 * n23  # Omega_f is the whole face
+    K_all = eqd.fault_stiffness(fe)
+    it_all = eqd.solver_report(fe.rs)["iterations"]
+    k = np.arange(nf)
+    ia, ib = k % n23, k // n23
+    zero = np.concatenate([(ib == 0) | (ib == n23 - 1), (ia == 0) | (ia == n23 - 1)])
+    zc, nzc = np.flatnonzero(zero), np.flatnonzero(~zero)
+    assert len(zc) == 4 * n23
+    fz, _ = pair("ring", precision="mixed", tile_k=64)
+    assert np.array_equal(eqd.fault_stiffness(fz, cols=zc), K_all[:, zc])
+    assert eqd.solver_report(fz.rs)["iterations"] == 0
+    fn, _ = pair("ring", precision="mixed", tile_k=64)
+    assert np.array_equal(eqd.fault_stiffness(fn, cols=nzc), K_all[:, nzc])
+    assert eqd.solver_report(fn.rs)["iterations"] == it_all
+    assert rel(K_all[:, zc], oracle_K("ring")[:, zc]) < 1e-8
+
+
 @pytest.mark.parametrize("name", ["ring", "long"])
 @pytest.mark.parametrize("precond", ["none", "jacobi"])
 def test_x1_inner_tiles_match_node_major(monkeypatch, name, precond):
