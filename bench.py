@@ -599,8 +599,8 @@ def main():
         copy_calibration(args.copy_calib_mib)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the K build shards by columns "
-              f"across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the K build would shard by "
+              f"columns across GPUs, one process per GPU; only the one-GPU path is built)", file=sys.stderr)
         sys.exit(2)
 
     mu, nu = 32.04e9, 0.25

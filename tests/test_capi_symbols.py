@@ -45,7 +45,7 @@ def test_operator_create_rejects_bad_patch_before_touching_device():
         eqd.FaultElasticity(eqd.GridPair(13, 13, 1.2, 1.2), 2.0, 1.0, l_f=0.55)
     assert e.value.code == _lib.EQD_ERR_GRID
     with pytest.raises(eqd.EqdError) as e:
-        eqd.bp8_grid_pair(100.0, 800.0, 400.0)   # BASELINE c1 as written: n1 = 5 < 9
+        eqd.bp8_grid_pair(100.0, 800.0, 400.0)   # dz = 100 m, L = (800, 400): n1 = 5 < 9
     assert e.value.code == _lib.EQD_ERR_GRID
 
 

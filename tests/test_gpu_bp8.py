@@ -138,7 +138,7 @@ def test_run_100h_matches_oracle(toeplitz_K):
     print(f"  V_max: max rel diff {d.max():.2e}, oracle spread max {spread.max():.2e}, saves held to 1e-6: "
           f"{int((bound == 1e-6).sum())} of {len(bound)}", flush=True)
     assert np.all(d <= bound)
-    assert np.all(bound == 1e-6)  # over the first 100 h the north-star bar applies at every save
+    assert np.all(bound == 1e-6)  # over the first 100 h the 1e-6 bar applies at every save
 
 
 def test_run_30day_published_gs(toeplitz_K):

@@ -1,7 +1,7 @@
 """Parity of the HIP path (through the C ABI) with the CPU oracle.
 
 Bars: operators (A, HP(D+SAT)chi, traction map) to 1e-13 relative; solutions and K
-within 1e-8 relative (Frobenius) of the oracle's FP64 CG, the north-star bound for
+within 1e-8 relative (Frobenius) of the oracle's FP64 CG, the stated bound for
 K; zero slip -> exactly zero traction (test/fault_response_test.jl:74-78); batched
 builds bitwise independent of how columns are batched (the reference's threaded ==
 serial property, test/fault_response_test.jl:100-112)."""
