@@ -234,6 +234,8 @@ void slip_fields(const eqd_op& op, const double* s2, const double* s3, double* f
 // Per-solver device workspace for one batch shape.
 struct Workspace {
     int K = 0, ntiles = 0, prec = -1;
+    int used_tiles = 0;  // > 0: the current batch fills only tiles [0, used_tiles) (set by run_columns)
+    int live_tiles() const { return used_tiles > 0 ? used_tiles : ntiles; }
     size_t nvec = 0;
     DevMem b, x, r, p, q, xc, fields, partial, folded, cs, tile_on, tile_rel, colid, outpos;
     int* h_tile_on = nullptr;
@@ -555,7 +557,8 @@ struct Engine {
     static void init_state(eqd_solver* s, hipStream_t st, const int* mask = nullptr) {
         Workspace& w = *s->ws;
         const eqd_op& op = *s->op;
-        dim3 grid(s->op->G, w.ntiles);
+        // b of the tiles the batch fills; the others hold idle columns, whose state ignores it
+        dim3 grid(s->op->G, w.live_tiles());
         if (xi(s))
             hipLaunchKernelGGL(k_xi_selfdot, grid, dim3(kBlock), 0, st, op.tb.NX, op.tb.NY, op.tb.mface,
                                w.b.as<double>(), w.partial.as<double>(),
@@ -566,7 +569,7 @@ struct Engine {
                                s->o.precond == EQD_PRECOND_JACOBI ? op.dinv64 : nullptr);
         const int64_t itmax = s->o.itmax > 0 ? s->o.itmax : 2 * op.ntot;
         if (op.nfold)
-            hipLaunchKernelGGL((k_fold_partials<K>), dim3(op.nfold, w.ntiles), dim3(K * 16), 0, st, op.G, op.fchunk,
+            hipLaunchKernelGGL((k_fold_partials<K>), dim3(op.nfold, w.live_tiles()), dim3(K * 16), 0, st, op.G, op.fchunk,
                                (const double*)w.partial.as<double>(), w.folded.as<double>());
         hipLaunchKernelGGL((k_init_state<K>), dim3(w.ntiles), dim3(K), 0, st, op.nfold ? op.nfold : op.G,
                            w.cs.as<ColState>(), op.nfold ? w.folded.as<double>() : w.partial.as<double>(),
@@ -606,7 +609,7 @@ struct Engine {
             tile_iters += act;
         }
         if (std::is_same<T, float>::value) {
-            dim3 grid(s->op->G, w.ntiles);
+            dim3 grid(s->op->G, w.live_tiles());
             xaccum_all(s, st, grid, nullptr);
             HIPCHK(hipGetLastError());
         }
@@ -683,7 +686,7 @@ struct Engine {
             tile_iters += (int64_t)act * gi;
         }
         if (std::is_same<T, float>::value) {  // fold any correction still held in FP32
-            dim3 grid(s->op->G, w.ntiles);
+            dim3 grid(s->op->G, w.live_tiles());
             xaccum_all(s, st, grid, nullptr);
             HIPCHK(hipGetLastError());
         }
@@ -719,7 +722,7 @@ struct Engine {
             ++chunks;
         }
         if (std::is_same<T, float>::value) {  // fold any correction still held in FP32
-            dim3 grid(s->op->G, w.ntiles);
+            dim3 grid(s->op->G, w.live_tiles());
             xaccum_all(s, st, grid, nullptr);
             HIPCHK(hipGetLastError());
         }
@@ -805,7 +808,7 @@ struct Engine {
     static void rhs_from_fields(eqd_solver* s, const int* mask = nullptr) {
         Workspace& w = *s->ws;
         const eqd_op& op = *s->op;
-        dim3 grid(s->op->G, w.ntiles);
+        dim3 grid(s->op->G, w.live_tiles());
         if (xi(s))
             hipLaunchKernelGGL(k_xi_rhs, grid, dim3(kBlock), 0, s->stream, op.rv, w.fields.as<double>(), w.ntiles * K,
                                w.b.as<double>(), w.x.as<double>(), w.r.as<float>(), w.p.as<float>(),
@@ -956,8 +959,13 @@ void run_columns(eqd_solver* s, const std::vector<int64_t>& colid, const std::ve
     std::vector<std::pair<int, int>> batches;  // [base, end) of each batch
     for (int b = 0; b < split; b += slots) batches.emplace_back(b, std::min(b + slots, split));
     for (int b = split; b < n; b += slots) batches.emplace_back(b, std::min(b + slots, n));
+    struct UsedTiles {  // the tiles a batch fills; every tile again afterwards
+        Workspace& w;
+        ~UsedTiles() { w.used_tiles = 0; }
+    } used_guard{w};
     for (const auto& bt : batches) {
         const int base = bt.first, end = bt.second;
+        w.used_tiles = (end - base + K - 1) / K;
         std::vector<int64_t> cid(slots, -1), opos(slots, -1);
         for (int i = 0; i < slots && base + i < end; ++i) {
             cid[i] = colid[base + i];
