@@ -993,6 +993,7 @@ __global__ __launch_bounds__(kBlock) void k_xi_rhs(RhsView rv, const double* __r
             const int m = sl.n + 16 * s;
             if (m >= sl.B) break;
             const size_t e = off + (size_t)bl * sl.blk + 4 * m;
+            double vals[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (!on[j]) continue;
@@ -1019,11 +1020,30 @@ __global__ __launch_bounds__(kBlock) void k_xi_rhs(RhsView rv, const double* __r
                         }
                     }
                 }
-                b[e + j] = val;
-                x[e + j] = 0.0;
-                r[e + j] = (float)val;
-                p[e + j] = (m < NX && dinv) ? (float)(dinv[(size_t)(m + NX * yz) * 3 + c] * val) : (float)val;
-                xc[e + j] = 0.0f;
+                vals[j] = val;
+            }
+            const double dz = (m < NX && dinv) ? dinv[(size_t)(m + NX * yz) * 3 + c] : 1.0;
+            if (on[0] && on[1] && on[2] && on[3]) {
+                // the four columns' entries are 32 (FP64) and 16 (FP32) contiguous bytes
+                *reinterpret_cast<xv4<double>*>(b + e) = xv4<double>{vals[0], vals[1], vals[2], vals[3]};
+                *reinterpret_cast<xv4<double>*>(x + e) = xv4<double>{0.0, 0.0, 0.0, 0.0};
+                *reinterpret_cast<xv4<float>*>(r + e) =
+                    xv4<float>{(float)vals[0], (float)vals[1], (float)vals[2], (float)vals[3]};
+                *reinterpret_cast<xv4<float>*>(p + e) =
+                    dinv ? xv4<float>{(float)(dz * vals[0]), (float)(dz * vals[1]), (float)(dz * vals[2]),
+                                      (float)(dz * vals[3])}
+                         : xv4<float>{(float)vals[0], (float)vals[1], (float)vals[2], (float)vals[3]};
+                *reinterpret_cast<xv4<float>*>(xc + e) = xv4<float>{0.0f, 0.0f, 0.0f, 0.0f};
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (!on[j]) continue;
+                    b[e + j] = vals[j];
+                    x[e + j] = 0.0;
+                    r[e + j] = (float)vals[j];
+                    p[e + j] = dinv ? (float)(dz * vals[j]) : (float)vals[j];
+                    xc[e + j] = 0.0f;
+                }
             }
         }
     }
