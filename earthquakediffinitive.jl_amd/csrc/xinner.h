@@ -314,6 +314,17 @@ __global__ __launch_bounds__(64 * (MODE == 0 ? EQD_XI_BW : 4)) __attribute__((am
     tile = (int)gridDim.y - 1 - tile;
 #endif
     if (!tile_on[tile]) return;
+#if EQD_XI_SLAB
+    if constexpr (MODE == 1) {
+        // the FP64 residual: a block whose 16-column slab holds no column flagged for a reliable
+        // update leaves at once (its partial slots zeroed; only flagged columns read theirs)
+        const int l = threadIdx.x & 63, sl = g / ((int)gridDim.x >> 2);
+        if (!__any(l < 16 && cs[(size_t)tile * 64 + 16 * sl + l].rel_flag != 0)) {
+            if (threadIdx.x < 64) partial[((size_t)tile * gridDim.x + g) * 64 + threadIdx.x] = 0.0;
+            return;
+        }
+    }
+#endif
     constexpr int BW = MODE == 0 ? EQD_XI_BW : 4;  // waves per block
     static_assert(BW == 4 || EQD_XI_SLAB, "other block sizes take the slab-major order");
     constexpr int WS = 2 * kRxS + 1, WO = 2 * kRxO + 1;
